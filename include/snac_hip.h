@@ -1,0 +1,224 @@
+/*
+ * snac_hip.h -- C ABI of the MI355X (gfx950) implementation of SNAC's explicit
+ * time step.
+ *
+ * The library (libsnac_hip.so) owns one rank's subdomain on one GPU: node
+ * coordinates and velocities, the per-tetrahedron stress tensors and the
+ * per-tetrahedron plastic strain / viscosity extensions.  The host keeps the
+ * reference's StGermain context, entry points and plugin registration; a
+ * "SnacHip" plugin replaces the default hooks of the loop-level entry points
+ * with the calls below.
+ *
+ * Reference interfaces each entry point stands in for are cited per function
+ * (paths relative to the geodynamics/snac tree).  All functions return an int
+ * status: SNH_OK (0) or a negative SNH_E* code; snh_last_error() gives the
+ * message of the last failure on the calling thread.  Nothing here aborts the
+ * process (the reference's Journal_Firewall does; StGermain/Base/IO/src/
+ * Journal.c:568-660).
+ *
+ * Threading: a context is used by one host thread at a time.  All device work
+ * of a context is issued on the context's stream (snh_set_stream); every entry
+ * point that returns host-visible values synchronises that stream.
+ */
+#ifndef SNAC_HIP_H
+#define SNAC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNH_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------- */
+#define SNH_OK            0
+#define SNH_EINVAL       -1   /* bad argument or configuration            */
+#define SNH_ENOMEM       -2   /* device or host allocation failed          */
+#define SNH_EHIP         -3   /* a HIP runtime call failed                 */
+#define SNH_ESTATE       -4   /* call out of order (e.g. step before zero) */
+#define SNH_EUNSUPPORTED -5   /* option the device path does not implement */
+#define SNH_ENUMERIC     -6   /* NaN/Inf firewall tripped                   */
+
+/* ---- fixed sizes (Snac/libSnac/src/TetrahedraTables.h:47-49) ---------- */
+#define SNH_TETRA_COUNT        10
+#define SNH_ELEMENT_NODE_COUNT  8
+#define SNH_MAX_SEGMENTS        8
+#define SNH_MAX_MATERIALS      16
+#define SNH_STRESS_COMPONENTS   6   /* order: 00, 11, 22, 01, 02, 12 */
+
+/* ---- rheology bits (Snac/libSnac/src/Material.h:46-50) ----------------- */
+#define SNH_RHEOLOGY_ELASTIC      0x1u
+#define SNH_RHEOLOGY_PLASTIC      0x2u
+#define SNH_RHEOLOGY_MAXWELL      0x4u
+#define SNH_RHEOLOGY_VISCOPLASTIC 0x8u
+
+/* ---- dt types (Snac/libSnac/src/Context.c:345-391) --------------------- */
+#define SNH_DT_CONSTANT 0
+#define SNH_DT_DYNAMIC  1
+#define SNH_DT_WAVE     2
+
+/* ---- velocity-BC node sets (StGermain/Discretisation/Utils/src/WallVC.c:581-670,
+ *      AllNodesVC).  I = x, J = y (vertical, top = max J), K = z.            */
+#define SNH_WALL_FRONT  0   /* k == 0     */
+#define SNH_WALL_BACK   1   /* k == K-1   */
+#define SNH_WALL_LEFT   2   /* i == 0     */
+#define SNH_WALL_RIGHT  3   /* i == I-1   */
+#define SNH_WALL_TOP    4   /* j == J-1   */
+#define SNH_WALL_BOTTOM 5   /* j == 0     */
+#define SNH_WALL_ALL    6   /* every node */
+
+/* component mask bits of a velocity condition */
+#define SNH_VX 0x1u
+#define SNH_VY 0x2u
+#define SNH_VZ 0x4u
+
+/* ---- arithmetic of the step -------------------------------------------- */
+#define SNH_PRECISION_FP64  0   /* FP64 throughout, reference-identical order */
+#define SNH_PRECISION_MIXED 1   /* FP32 strain-rate/constitutive/force, FP64 state */
+
+/* ---- per-element input fields (generalisations of per-material values) - */
+#define SNH_FIELD_REFVISC          0  /* reference viscosity [Pa s], replaces material->refvisc */
+#define SNH_FIELD_COHESION_OFFSET  1  /* [Pa], added to the softened cohesion                   */
+
+/* One material phase: the subset of Snac_Material (Snac/libSnac/src/Material.h:53-100)
+ * the explicit step reads.  Units SI; angles in degrees. */
+typedef struct snh_material {
+    uint32_t rheology;            /* SNH_RHEOLOGY_* bits                      */
+    double   phsDensity;
+    double   alpha, beta;         /* density( T, p ) coefficients             */
+    double   lambda, mu;          /* Lame parameters                          */
+    uint32_t nsegments;           /* piecewise-linear softening segments      */
+    double   plstrain[SNH_MAX_SEGMENTS + 1];
+    double   frictionAngle[SNH_MAX_SEGMENTS + 1];
+    double   dilationAngle[SNH_MAX_SEGMENTS + 1];
+    double   cohesion[SNH_MAX_SEGMENTS + 1];
+    double   ten_off;             /* tension cut-off; 0 => cohesion/tan(phi)  */
+    double   vis_min, vis_max, refvisc, refsrate, reftemp;
+    double   activationE, srexponent;
+} snh_material;
+
+/* One velocity condition of the velocityBCs composite (StGermain/Base/Automation/
+ * src/CompositeVC.c).  Later entries win where two constrain one component
+ * (_CompositeVC_GetValueIndex searches items last to first). */
+typedef struct snh_velocity_bc {
+    int32_t  wall;                /* SNH_WALL_*                               */
+    uint32_t mask;                /* SNH_VX | SNH_VY | SNH_VZ                 */
+    double   value[3];            /* [m/s] for the masked components          */
+} snh_velocity_bc;
+
+/* Problem description: the mesh, phase-independent constants and BCs that
+ * _Snac_Context_Init / _Snac_Context_InitialConditions read from the
+ * dictionary (Snac/libSnac/src/Context.c:155-204,345-391,742-824). */
+typedef struct snh_config {
+    uint32_t meshSizeI, meshSizeJ, meshSizeK;   /* NODE counts per axis (mesh struct) */
+    double   minX, minY, minZ, maxX, maxY, maxZ; /* BlockGeometry extent               */
+    int32_t  dtType;              /* SNH_DT_*                                  */
+    double   timeStep;            /* initial dt (constant: the dt)              */
+    double   gravity;             /* acts on -y (Snac/libSnac/src/Force.c:173)  */
+    double   demf;                /* local damping factor                       */
+    double   strain_inert;        /* dynamic mass-scaling factor                */
+    double   topo_kappa;          /* must be 0: surface diffusion is not on the path */
+    uint32_t nmaterials;
+    const snh_material* materials;
+    uint32_t nbcs;
+    const snh_velocity_bc* bcs;
+    int32_t  precision;           /* SNH_PRECISION_*                            */
+} snh_config;
+
+/* Host-side view of the step state, in the reference's index order.
+ * Any pointer may be NULL to skip that field.
+ *   coord          [N][3]        mesh->nodeCoord
+ *   velocity       [N][3]        Snac_Node.velocity
+ *   stress         [E][10][6]    Snac_Element_Tetrahedra.stress (00,11,22,01,02,12)
+ *   plastic_strain [E][10]       SnacPlastic_Element.plasticStrain
+ *   viscosity      [E][10]       SnacMaxwell_Element.viscosity (download only)
+ *   volume         [E][10]       tetra volume of the current geometry (download only)
+ * Node index n = i + I*(j + J*k); element index e = i + (I-1)*(j + (J-1)*k)
+ * (StGermain/Discretisation/Mesh/src/HexaEL.c:400-426). */
+typedef struct snh_state {
+    double* coord;
+    double* velocity;
+    double* stress;
+    double* plastic_strain;
+    double* viscosity;
+    double* volume;
+} snh_state;
+
+/* Step scalars kept by the context (Snac_Context fields). */
+typedef struct snh_scalars {
+    double   dt;                  /* dt of the next step                       */
+    double   speedOfSound;
+    double   minLengthScale;
+    double   vmax;                /* max |v| over constrained nodes, floored    */
+    uint32_t timeStep;            /* steps completed                           */
+} snh_scalars;
+
+typedef struct snh_context snh_context;
+
+/* Library version / last error. */
+int         snh_abi_version(void);
+const char* snh_last_error(void);
+
+/* Create a context for one subdomain and allocate its device state.  Builds
+ * the mesh (BlockGeometry.c:326-349 coordinates, zero velocity/stress) and the
+ * per-node BC table.  Replaces: Snac_Context_New + _Snac_Context_Build
+ * (Snac/libSnac/src/Context.c:90-100,707-740). */
+int snh_context_create(const snh_config* cfg, snh_context** out);
+int snh_context_destroy(snh_context* ctx);
+
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL = the
+ * context's own stream. */
+int snh_set_stream(snh_context* ctx, void* stream);
+
+/* Per-element material index (element->material_I).  nelem entries. */
+int snh_set_element_material(snh_context* ctx, const uint8_t* material, size_t nelem);
+
+/* Per-element input field (SNH_FIELD_*), nelem doubles. */
+int snh_set_element_field(snh_context* ctx, int field, const double* values, size_t nelem);
+
+/* Copy state in/out (host memory).  Upload also re-applies nothing: the
+ * caller passes the state as the reference holds it after
+ * _Snac_Context_InitialConditions (Context.c:742-774). */
+int snh_upload_state(snh_context* ctx, const snh_state* state);
+int snh_download_state(snh_context* ctx, snh_state* state);
+
+/* Initial isotropic lithostatic stress of the SnacHydroStaticIC plugin
+ * (Snac/plugins/hydroStaticIC/VariableConditions.c:43-160, hooked before
+ * SnacTimeStepZero at Register.c:127-132), for one rank: per element column,
+ * top down, P = -(rogh + dP/2) on the diagonal of every tet stress.  Host-side
+ * computation on the current coordinates, then uploaded. */
+int snh_hydrostatic_ic(snh_context* ctx);
+
+/* Snac_Context_TimeStepZero (Snac/libSnac/src/Context.c:978-1036): geometry of
+ * the initial mesh, minLengthScale and the first Sync (dt). */
+int snh_timestep_zero(snh_context* ctx);
+
+/* The loop-level entry points of one step, callable one by one from the
+ * host's hooks:
+ *   snh_calc_stresses            Snac_EP_CalcStresses      (Context.c:1102-1117)
+ *   snh_loop_nodes_momentum      Snac_EP_LoopNodesMomentum (Context.c:1120-1154)
+ *   snh_loop_elements_momentum   Snac_EP_LoopElementsMomentum (Context.c:1156-1184)
+ *   snh_sync                     AbstractContext_EP_Sync -> _Snac_Context_Sync
+ *                                                          (Context.c:1208-1268)
+ * The Jaumann rotation computed by LoopElementsMomentum is applied to the
+ * stored stress at the start of the next CalcStresses (same values, same
+ * order); snh_download_state returns the rotated stress in between. */
+int snh_calc_stresses(snh_context* ctx);
+int snh_loop_nodes_momentum(snh_context* ctx);
+int snh_loop_elements_momentum(snh_context* ctx);
+int snh_sync(snh_context* ctx);
+
+/* nsteps whole steps (AbstractContext_Step, StGermain/Base/Context/src/
+ * AbstractContext.c:921-944) with dt kept on the device.  Asynchronous:
+ * returns after enqueueing; snh_synchronize or any download waits. */
+int snh_step(snh_context* ctx, int nsteps);
+
+int snh_get_scalars(snh_context* ctx, snh_scalars* out);
+int snh_synchronize(snh_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNAC_HIP_H */

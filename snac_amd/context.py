@@ -1,0 +1,138 @@
+"""Host-side driver of the explicit step on one GPU.
+
+Mirrors the lifecycle of the reference's Python Context
+(Snac/Python/Context.py; Snac/tests/basic-python.py):
+Construct -> Build -> Initialise -> Execute -> Destroy, with the explicit
+step itself running behind the C ABI of libsnac_hip.so (include/snac_hip.h).
+There is no CPU fallback: if the library cannot be loaded, construction fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _abi
+from .problem import Problem
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One subdomain (one rank's mesh) on one GPU."""
+
+    def __init__(self, problem: Problem):
+        self.problem = problem
+        self.lib = None
+        self.h = None
+        self.E = problem.n_elements
+        self.N = problem.n_nodes
+        self._keep = None
+
+    # -- lifecycle (Stg_Component_Construct / Build / Initialise / Execute) --
+    def Construct(self):
+        self.lib = _abi.load_library()
+        cfg, self._keep = self.problem.to_c()
+        h = C.c_void_p()
+        self._check(self.lib.snh_context_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        p = self.problem
+        if p.element_material is not None:
+            m = np.ascontiguousarray(p.element_material, dtype=np.uint8)
+            self._check(self.lib.snh_set_element_material(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8)), m.size))
+        for fid, vals in p.fields.items():
+            v = np.ascontiguousarray(vals, dtype=np.float64)
+            self._check(self.lib.snh_set_element_field(self.h, fid, _dp(v), v.size))
+        return self
+
+    def Build(self):
+        return self
+
+    def Initialise(self):
+        """Initial conditions (plastic-strain seeds) then TimeStepZero."""
+        p = self.problem
+        if p.hydrostatic:
+            self._check(self.lib.snh_hydrostatic_ic(self.h))
+        if p.plastic_strain0 is not None:
+            ps = np.ascontiguousarray(p.plastic_strain0, dtype=np.float64).reshape(self.E, 10)
+            st = _abi.State()
+            st.plastic_strain = _dp(ps)
+            self._check(self.lib.snh_upload_state(self.h, C.byref(st)))
+        self._check(self.lib.snh_timestep_zero(self.h))
+        return self
+
+    def Execute(self, nsteps: int):
+        self.step(nsteps)
+        self.synchronize()
+        return self
+
+    def Destroy(self):
+        if self.h is not None and self.lib is not None:
+            self.lib.snh_context_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self.Construct().Build().Initialise()
+
+    def __exit__(self, *exc):
+        self.Destroy()
+
+    def __del__(self):
+        try:
+            self.Destroy()
+        except Exception:
+            pass
+
+    # -- stepping ------------------------------------------------------------
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.snh_step(self.h, int(nsteps)))
+
+    def calc_stresses(self):
+        self._check(self.lib.snh_calc_stresses(self.h))
+
+    def loop_nodes_momentum(self):
+        self._check(self.lib.snh_loop_nodes_momentum(self.h))
+
+    def loop_elements_momentum(self):
+        self._check(self.lib.snh_loop_elements_momentum(self.h))
+
+    def sync(self):
+        self._check(self.lib.snh_sync(self.h))
+
+    def synchronize(self):
+        self._check(self.lib.snh_synchronize(self.h))
+
+    # -- state ----------------------------------------------------------------
+    def scalars(self) -> Dict[str, float]:
+        s = _abi.Scalars()
+        self._check(self.lib.snh_get_scalars(self.h, C.byref(s)))
+        return {"dt": s.dt, "speedOfSound": s.speedOfSound, "minLengthScale": s.minLengthScale,
+                "vmax": s.vmax, "timeStep": s.timeStep}
+
+    def state(self, fields=("coord", "velocity", "stress", "plastic_strain", "viscosity")) -> Dict[str, np.ndarray]:
+        shapes = {"coord": (self.N, 3), "velocity": (self.N, 3), "stress": (self.E, 10, 6),
+                  "plastic_strain": (self.E, 10), "viscosity": (self.E, 10), "volume": (self.E, 10)}
+        out = {f: np.zeros(shapes[f]) for f in fields}
+        st = _abi.State()
+        for f, a in out.items():
+            setattr(st, f, _dp(a))
+        self._check(self.lib.snh_download_state(self.h, C.byref(st)))
+        return out
+
+    def set_state(self, coord=None, velocity=None, stress=None, plastic_strain=None):
+        keep = []
+        st = _abi.State()
+        for name, a, shape in (("coord", coord, (self.N, 3)), ("velocity", velocity, (self.N, 3)),
+                               ("stress", stress, (self.E, 10, 6)), ("plastic_strain", plastic_strain, (self.E, 10))):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
+            keep.append(a)
+            setattr(st, name, _dp(a))
+        self._check(self.lib.snh_upload_state(self.h, C.byref(st)))
+
+    def _check(self, rc):
+        return _abi.check(rc, self.lib)

@@ -1,0 +1,1529 @@
+/*
+ * snac_hip.hip -- gfx950 kernels and C ABI (include/snac_hip.h) of SNAC's
+ * explicit time step, FP64 path.
+ *
+ * One step is four launches on the context's stream:
+ *   k_elem   per element: geometry of the current mesh, the pending Jaumann
+ *            rotation of the previous step (UpdateElement.c:145-265), strain
+ *            rate (StrainRate.c), strain, constitutive update and trace
+ *            averaging (Stress.c).  Geometry and velocity gradient are computed
+ *            once and serve both the rotation and the strain rate, whose inputs
+ *            are identical in the reference.
+ *   k_node   per node: atomic-free gather over the 8 surrounding elements x 5
+ *            tets in the reference's order (Force.c:87-183), damping, velocity,
+ *            velocity BC and coordinate update (UpdateNode.c:60-158), and the
+ *            block maximum of |v| over constrained nodes.
+ *   k_geo    per element: tet volumes and face areas of the new mesh and the
+ *            block minimum of V/A (UpdateElement.c:84-139).
+ *   k_fin    one block: global min/max and the dynamic dt (Context.c:1208-1268).
+ * dt stays on the device, so a run of steps needs no host synchronisation.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/snac_hip.h"
+#include "snh_kernels.h"
+
+using namespace snh;
+
+__constant__ DevMaterial c_mat[SNH_MAX_MATERIALS];
+__constant__ DevBC c_bc[256];
+
+#define ELEM_BLOCK 128
+#define NODE_BLOCK 128
+#define GEO_BLOCK 256
+#define FIN_BLOCK 1024
+
+/* --------------------------------------------------------------------------
+ * helpers
+ * ------------------------------------------------------------------------ */
+__device__ __forceinline__ int node_index(const StepArgs& a, int i, int j, int k) {
+    return i + a.I * (j + a.J * k);
+}
+
+/* load the 8 hex nodes of element (i,j,k): coords and velocities */
+__device__ __forceinline__ void load_hex(const StepArgs& a, const double* __restrict__ x, int i, int j, int k,
+                                         double X[8][3]) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        int n = node_index(a, i + kHexOff[q][0], j + kHexOff[q][1], k + kHexOff[q][2]);
+        X[q][0] = x[n];
+        X[q][1] = x[a.N + n];
+        X[q][2] = x[2 * a.N + n];
+    }
+}
+
+__device__ __forceinline__ double block_min(double v, double* sh) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int nw = (blockDim.x + 63) >> 6;
+    if (wid == 0) {
+        v = lane < nw ? sh[lane] : INFINITY;
+        for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    }
+    return v;
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int nw = (blockDim.x + 63) >> 6;
+    if (wid == 0) {
+        v = lane < nw ? sh[lane] : -INFINITY;
+        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    }
+    return v;
+}
+
+/* --------------------------------------------------------------------------
+ * eigen-decomposition of plastic_BI (Constitutive.c:240-510): Householder
+ * tridiagonalisation and implicit QL for n = 3 with the reference's int abs()
+ * (EISPACK tred2/tql2 as restated there).
+ * ------------------------------------------------------------------------ */
+__device__ void tred2_3(double d[3], double e[3], double V[3][3]) {
+    const int n = 3;
+#pragma unroll
+    for (int j = 0; j < n; j++) d[j] = V[n - 1][j];
+#pragma unroll
+    for (int i = n - 1; i > 0; i--) {
+        double scale = 0.0, h = 0.0;
+#pragma unroll
+        for (int k = 0; k < i; k++) scale = scale + snh_ref_iabs(d[k]);
+        if (scale == 0.0) {
+            e[i] = d[i - 1];
+#pragma unroll
+            for (int j = 0; j < i; j++) {
+                d[j] = V[i - 1][j];
+                V[i][j] = 0.0;
+                V[j][i] = 0.0;
+            }
+        } else {
+            double f, g, hh;
+#pragma unroll
+            for (int k = 0; k < i; k++) {
+                d[k] /= scale;
+                h += d[k] * d[k];
+            }
+            f = d[i - 1];
+            g = sqrt(h);
+            if (f > 0) g = -g;
+            e[i] = scale * g;
+            h = h - f * g;
+            d[i - 1] = f - g;
+#pragma unroll
+            for (int j = 0; j < i; j++) e[j] = 0.0;
+#pragma unroll
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                V[j][i] = f;
+                g = e[j] + V[j][j] * f;
+#pragma unroll
+                for (int k = j + 1; k <= i - 1; k++) {
+                    g += V[k][j] * d[k];
+                    e[k] += V[k][j] * f;
+                }
+                e[j] = g;
+            }
+            f = 0.0;
+#pragma unroll
+            for (int j = 0; j < i; j++) {
+                e[j] /= h;
+                f += e[j] * d[j];
+            }
+            hh = f / (h + h);
+#pragma unroll
+            for (int j = 0; j < i; j++) e[j] -= hh * d[j];
+#pragma unroll
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                g = e[j];
+#pragma unroll
+                for (int k = j; k <= i - 1; k++) V[k][j] -= (f * e[k] + g * d[k]);
+                d[j] = V[i - 1][j];
+                V[i][j] = 0.0;
+            }
+        }
+        d[i] = h;
+    }
+#pragma unroll
+    for (int i = 0; i < n - 1; i++) {
+        double h, g;
+        V[n - 1][i] = V[i][i];
+        V[i][i] = 1.0;
+        h = d[i + 1];
+        if (h != 0.0) {
+#pragma unroll
+            for (int k = 0; k <= i; k++) d[k] = V[k][i + 1] / h;
+#pragma unroll
+            for (int j = 0; j <= i; j++) {
+                g = 0.0;
+#pragma unroll
+                for (int k = 0; k <= i; k++) g += V[k][i + 1] * V[k][j];
+#pragma unroll
+                for (int k = 0; k <= i; k++) V[k][j] -= g * d[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k <= i; k++) V[k][i + 1] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < n; j++) {
+        d[j] = V[n - 1][j];
+        V[n - 1][j] = 0.0;
+    }
+    V[n - 1][n - 1] = 1.0;
+    e[0] = 0.0;
+}
+
+/* QL sweeps on a 3x3 tridiagonal; the column index m and the sweep loop are
+   data dependent, so V is updated through a column-pair helper that keeps
+   every array index static. */
+__device__ __forceinline__ void rot_cols(double V[3][3], int i, double s, double c) {
+    /* columns i and i+1, i in {0,1} */
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double vi = (i == 0) ? V[k][0] : V[k][1];
+        double vi1 = (i == 0) ? V[k][1] : V[k][2];
+        double h = vi1;
+        double n1 = s * vi + c * h;
+        double n0 = c * vi - s * h;
+        if (i == 0) {
+            V[k][1] = n1;
+            V[k][0] = n0;
+        } else {
+            V[k][2] = n1;
+            V[k][1] = n0;
+        }
+    }
+}
+
+__device__ __forceinline__ double sel3(const double a[3], int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+__device__ __forceinline__ void set3(double a[3], int i, double v) {
+    if (i == 0) a[0] = v;
+    else if (i == 1) a[1] = v;
+    else a[2] = v;
+}
+
+__device__ void tql2_3(double d[3], double e[3], double V[3][3]) {
+    const int n = 3;
+    double f = 0.0, tst1 = 0.0;
+    const double eps = 2.220446049250313080847e-16; /* pow(2.0,-52.0) */
+    e[0] = e[1];
+    e[1] = e[2];
+    e[2] = 0.0;
+#pragma unroll
+    for (int l = 0; l < n; l++) {
+        int m = l;
+        {
+            double b = (double)snh_iadd_wrap(snh_ref_iabs(d[l]), snh_ref_iabs(e[l]));
+            tst1 = (tst1 > b) ? tst1 : b;
+        }
+        while (m < n) {
+            if (snh_ref_iabs(sel3(e, m)) <= eps * tst1) break;
+            m++;
+        }
+        if (m > l) {
+            int guard = 0;
+            do {
+                double g = d[l];
+                double p = (d[l + 1] - g) / (2.0 * e[l]);
+                double r = snh_hypot(p, 1.0);
+                double dl1, h, cc, c2, c3, el1, s, s2;
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
+                dl1 = d[l + 1];
+                h = g - d[l];
+#pragma unroll
+                for (int i = l + 2; i < n; i++) d[i] -= h;
+                f = f + h;
+                p = sel3(d, m);
+                cc = 1.0;
+                c2 = cc;
+                c3 = cc;
+                el1 = e[l + 1];
+                s = 0.0;
+                s2 = 0.0;
+                for (int i = m - 1; i >= l; i--) {
+                    double ei = sel3(e, i), di = sel3(d, i);
+                    c3 = c2;
+                    c2 = cc;
+                    s2 = s;
+                    g = cc * ei;
+                    h = cc * p;
+                    r = snh_hypot(p, ei);
+                    set3(e, i + 1, s * r);
+                    s = ei / r;
+                    cc = p / r;
+                    p = cc * di - s * g;
+                    set3(d, i + 1, h + s * (cc * g + s * di));
+                    rot_cols(V, i, s, cc);
+                }
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = cc * p;
+                /* the reference loop has no iteration cap; a cap of 64
+                   sweeps (QL converges in a few) only guards the device */
+            } while (snh_ref_iabs(e[l]) > eps * tst1 && ++guard < 64);
+        }
+        d[l] = d[l] + f;
+        e[l] = 0.0;
+    }
+    /* ascending sort with eigenvector swaps */
+#pragma unroll
+    for (int i = 0; i < n - 1; i++) {
+        double p = d[i];
+        int k = i;
+#pragma unroll
+        for (int j = i + 1; j < n; j++)
+            if (d[j] < p) {
+                k = j;
+                p = d[j];
+            }
+        if (k != i) {
+            set3(d, k, d[i]);
+            d[i] = p;
+#pragma unroll
+            for (int j = 0; j < n; j++) {
+                double q = V[j][i];
+                double vk = sel3(V[j], k);
+                V[j][i] = vk;
+                set3(V[j], k, q);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void principal_stresses(const double st[3][3], double d[3], double V[3][3]) {
+    double e[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) V[i][j] = st[i][j];
+    tred2_3(d, e, V);
+    tql2_3(d, e, V);
+}
+
+/* piecewise-linear softening (plastic_BI/Constitutive.c:116-132,
+   viscoplastic_BI/Constitutive.c:266-280): values persist when no segment
+   matches */
+__device__ __forceinline__ void soften(const DevMaterial& m, double ps, double& phi, double& psi, double& coh,
+                                       double& hard) {
+    for (uint32_t i = 0; i < m.nseg; i++) {
+        const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
+        if (ps >= pl1 && ps <= pl2) {
+            const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
+            const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
+            const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+            phi = m.fa[i] + tgf * (ps - pl1);
+            psi = m.da[i] + tgd * (ps - pl1);
+            coh = m.coh[i] + tgc * (ps - pl1);
+            hard = tgc;
+        }
+    }
+}
+
+/* Mohr-Coulomb return in principal space and back-rotation
+   (plastic_BI/Constitutive.c:159-223; viscoplastic_BI :307-371 with the
+   hardening term).  st is the full 3x3 tensor; returns whether the tet
+   yielded, with the plastic strain increment in dps. */
+__device__ __forceinline__ bool mc_return(const DevMaterial& m, double st[3][3], double s[3], const double cn[3][3],
+                                          double sphi, double spsi, double cohesion, double tension_cutoff,
+                                          double hardening, bool with_hard, double& dps) {
+    const double a1 = m.lambda + 2.0 * m.mu;
+    const double a2 = m.lambda;
+    double alam, dep1, dep2, dep3;
+    double anphi = (1.0 + sphi) / (1.0 - sphi);
+    double anpsi = (1.0 + spsi) / (1.0 - spsi);
+    double fs = s[0] - s[2] * anphi + 2 * cohesion * sqrt(anphi);
+    double ft = s[2] - tension_cutoff;
+    if (!(fs < 0.0 || ft > 0.0)) return false;
+    double aP = sqrt(1.0 + anphi * anphi) + anphi;
+    double sP = tension_cutoff * anphi - 2 * cohesion * sqrt(anphi);
+    double h = s[2] - tension_cutoff + aP * (s[0] - sP);
+    if (h < 0.0) {
+        if (with_hard)
+            alam = fs / (a1 - a2 * anpsi + a1 * anphi * anpsi - a2 * anphi + 2.0 * sqrt(anphi) * hardening);
+        else
+            alam = fs / (a1 - a2 * anpsi + a1 * anphi * anpsi - a2 * anphi);
+        s[0] -= alam * (a1 - a2 * anpsi);
+        s[1] -= alam * a2 * (1.0 - anpsi);
+        s[2] -= alam * (a2 - a1 * anpsi);
+        dep1 = alam;
+        dep2 = 0.0;
+        dep3 = -alam * anpsi;
+    } else {
+        alam = ft / a1;
+        s[0] -= alam * a2;
+        s[1] -= alam * a2;
+        s[2] -= alam * a1;
+        dep1 = 0.0;
+        dep2 = 0.0;
+        dep3 = alam;
+    }
+    double depm = (dep1 + dep2 + dep3) / 3.0;
+    dps = sqrt(0.5 * ((dep1 - depm) * (dep1 - depm) + (dep2 - depm) * (dep2 - depm) +
+                      (dep3 - depm) * (dep3 - depm) + depm * depm));
+#pragma unroll
+    for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+        for (int nn = 0; nn < 3; nn++) st[mm][nn] = 0.0;
+#pragma unroll
+    for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+        for (int nn = mm; nn < 3; nn++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) st[mm][nn] += cn[mm][k] * cn[nn][k] * s[k];
+    return true;
+}
+
+/* sines/tangent of the softened angles: the host's libm values when the
+   angles equal the first table entry (constant-angle tables), the device's
+   otherwise */
+__device__ __forceinline__ void angle_terms(const DevMaterial& m, double phi, double psi, double& sphi, double& spsi) {
+    const double degrad = 0.017453292519943295; /* PI/180 */
+    sphi = (phi == m.fa[0]) ? m.sphi : sin(phi * degrad);
+    spsi = (psi == m.da[0]) ? m.spsi : sin(psi * degrad);
+}
+__device__ __forceinline__ double tan_phi(const DevMaterial& m, double phi) {
+    const double degrad = 0.017453292519943295;
+    return (phi == m.fa[0]) ? m.tanphi : tan(phi * degrad);
+}
+
+/* viscosity of maxwell (:123-151) / viscoplastic_BI (:174-218); the
+   temperature field is zero (SnacTemperature not on the path). */
+__device__ __forceinline__ double viscosity_of(const DevMaterial& m, double rvisc, double sd0, double sd1, double sd2,
+                                               const double sn[6], double dt) {
+    double visc;
+    if (m.linear_visc) {
+        visc = rvisc; /* pow(x, 0) * exp(0) == 1 exactly */
+    } else {
+        const double R = 8.31448;
+        double srJ2 = sqrt(fabs(sd1 * sd2 + sd2 * sd0 + sd0 * sd1 - sn[3] * sn[3] - sn[4] * sn[4] - sn[5] * sn[5])) / dt;
+        if (srJ2 == 0.0) srJ2 = m.refsrate;
+        double avgTemp = 0.0;
+        visc = rvisc * pow((srJ2 / m.refsrate), (1. / m.srexponent - 1.)) *
+               exp(m.activationE / R * (1. / (avgTemp + 273.15) - 1. / (m.reftemp + 273.15)));
+    }
+    if (visc < m.vis_min) visc = m.vis_min;
+    if (visc > m.vis_max) visc = m.vis_max;
+    return visc;
+}
+
+/* deviatoric Maxwell update shared by maxwell and viscoplastic_BI */
+__device__ __forceinline__ void maxwell_update(const DevMaterial& m, double st[6], const double sn[6],
+                                               double trace_strain, double visc, double dt) {
+    const double bulkm = m.lambda + 2.0 * m.mu / 3.0;
+    const double rmu = m.mu;
+    double trace_stress = st[0] + st[1] + st[2];
+    double straind0 = sn[0] - (trace_strain) / 3.0;
+    double straind1 = sn[1] - (trace_strain) / 3.0;
+    double straind2 = sn[2] - (trace_strain) / 3.0;
+    double stressd0 = st[0] - (trace_stress) / 3.0;
+    double stressd1 = st[1] - (trace_stress) / 3.0;
+    double stressd2 = st[2] - (trace_stress) / 3.0;
+    double temp = rmu / (2.0 * visc) * dt;
+    double vic1 = 1.0 - temp;
+    double vic2 = 1.0 / (1.0 + temp);
+    stressd0 = (stressd0 * vic1 + 2.0 * rmu * straind0) * vic2;
+    stressd1 = (stressd1 * vic1 + 2.0 * rmu * straind1) * vic2;
+    stressd2 = (stressd2 * vic1 + 2.0 * rmu * straind2) * vic2;
+    st[3] = (st[3] * vic1 + 2.0 * rmu * sn[3]) * vic2;
+    st[4] = (st[4] * vic1 + 2.0 * rmu * sn[4]) * vic2;
+    st[5] = (st[5] * vic1 + 2.0 * rmu * sn[5]) * vic2;
+    double VolumicStress = trace_stress / 3.0 + bulkm * trace_strain;
+    st[0] = stressd0 + VolumicStress;
+    st[1] = stressd1 + VolumicStress;
+    st[2] = stressd2 + VolumicStress;
+}
+
+/* deviatoric strains of a strain tensor for the viscosity law */
+__device__ __forceinline__ void devstrain(const double sn[6], double trace_strain, double& d0, double& d1, double& d2) {
+    d0 = sn[0] - (trace_strain) / 3.0;
+    d1 = sn[1] - (trace_strain) / 3.0;
+    d2 = sn[2] - (trace_strain) / 3.0;
+}
+
+/* --------------------------------------------------------------------------
+ * k_elem: Snac_EP_CalcStresses with the pending rotation
+ * ------------------------------------------------------------------------ */
+enum { RS_ELASTIC = 1, RS_PLASTIC = 2, RS_MAXWELL = 4, RS_VISCOPLASTIC = 8 };
+
+template <int RSET>
+__global__ void __launch_bounds__(ELEM_BLOCK) k_elem(StepArgs a) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    const double dt = a.scal->dt;
+    const double dt_rot = a.scal->dt_rot;
+    const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+
+    double X[8][3], U[8][3];
+    load_hex(a, a.xc, ei, ej, ek, X);
+    load_hex(a, a.v, ei, ej, ek, U);
+
+    /* pass 1: kinematics of the 10 tets */
+    double sr[NTET][6], w[NTET][3], vol[NTET];
+    double evol = 0.0;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        double Xt[4][3], Ut[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                Xt[q][d] = X[kTetraToNode[t][q]][d];
+                Ut[q][d] = U[kTetraToNode[t][q]][d];
+            }
+        TetKin k;
+        tet_kinematics(Xt, Ut, k);
+        vol[t] = k.V;
+        evol += k.V;
+#pragma unroll
+        for (int c = 0; c < 6; c++) sr[t][c] = k.sr[c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) w[t][c] = k.w[c];
+    }
+    evol *= 0.5;                        /* UpdateElement.c:267-268 */
+    const double elemVolume = 2.0 * evol; /* StrainRate.c:57, Stress.c:68 */
+    double partial = 0.0;
+    double trace[NTET];
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        trace[t] = (sr[t][0] + sr[t][1] + sr[t][2]) / 3.0;
+        partial += trace[t] * vol[t];
+    }
+    const double srcorr = partial / elemVolume;
+
+    /* pass 2: rotation, strain, constitutive */
+    double sig[NTET][6];
+    const size_t E = (size_t)a.E;
+    /* plastic_BI / viscoplastic_BI keep their softened parameters across the
+       tets of an element */
+    double pcoh = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0e6 : 0.0;
+    double pphi = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0 : 0.0;
+    double ppsi = 0.0, phard = 0.0, ptc = 0.0;
+    const double cohoff = a.cohoff ? a.cohoff[e] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        double st[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) st[c] = a.stress[((size_t)c * NTET + t) * E + e];
+        rotate_stress(st, w[t], dt_rot);
+        double s6[6];
+#pragma unroll
+        for (int c = 0; c < 3; c++) s6[c] = sr[t][c] + (-1.0 * trace[t] + srcorr);
+#pragma unroll
+        for (int c = 3; c < 6; c++) s6[c] = sr[t][c];
+        double sn[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) sn[c] = s6[c] * dt;
+
+        if ((RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC)) {
+            double plasticStrain = a.ps[(size_t)t * E + e];
+            double trace_strain = sn[0] + sn[1] + sn[2];
+            double S[3][3];
+            st[0] += (2.0 * m.mu) * sn[0] + m.lambda * (trace_strain);
+            st[1] += (2.0 * m.mu) * sn[1] + m.lambda * (trace_strain);
+            st[2] += (2.0 * m.mu) * sn[2] + m.lambda * (trace_strain);
+            st[3] += (2.0 * m.mu) * sn[3];
+            st[4] += (2.0 * m.mu) * sn[4];
+            st[5] += (2.0 * m.mu) * sn[5];
+            S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
+            S[0][1] = S[1][0] = st[3];
+            S[0][2] = S[2][0] = st[4];
+            S[1][2] = S[2][1] = st[5];
+            double d3[3], cn[3][3];
+            principal_stresses(S, d3, cn);
+            soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
+            double coh = a.cohoff ? pcoh + cohoff : pcoh;
+            if (pphi >= 0.0) {
+                ptc = m.ten_off;
+                if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
+            } else {
+                /* negative plastic strain is reset (plastic_BI :143-151); a
+                   negative friction angle otherwise is an error the host
+                   checks before the run */
+                plasticStrain = 0.0;
+                a.ps[(size_t)t * E + e] = 0.0;
+                pphi = m.fa[0];
+                ppsi = m.da[0];
+                pcoh = m.coh[0];
+                coh = a.cohoff ? pcoh + cohoff : pcoh;
+            }
+            double sphi, spsi;
+            angle_terms(m, pphi, ppsi, sphi, spsi);
+            double dps = 0.0;
+            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, 0.0, false, dps)) {
+                a.ps[(size_t)t * E + e] = plasticStrain + dps;
+                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            }
+        } else if ((RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)) {
+            double plasticStrain = a.ps[(size_t)t * E + e];
+            double trace_strain = sn[0] + sn[1] + sn[2];
+            double d0, d1, d2;
+            devstrain(sn, trace_strain, d0, d1, d2);
+            double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
+            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+            if (a.visc) a.visc[(size_t)t * E + e] = visc;
+            maxwell_update(m, st, sn, trace_strain, visc, dt);
+            /* unsymmetrised: the lower triangle is zero (viscoplastic_BI :246) */
+            double S[3][3] = {{st[0], st[3], st[4]}, {0.0, st[1], st[5]}, {0.0, 0.0, st[2]}};
+            double d3[3], cn[3][3];
+            principal_stresses(S, d3, cn);
+            soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
+            double coh = a.cohoff ? pcoh + cohoff : pcoh;
+            if (pphi >= 0.0) {
+                ptc = m.ten_off;
+                if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
+            } else {
+                plasticStrain = 0.0;
+                a.ps[(size_t)t * E + e] = 0.0;
+                pphi = m.fa[0];
+                ppsi = m.da[0];
+                pcoh = m.coh[0];
+                coh = a.cohoff ? pcoh + cohoff : pcoh;
+            }
+            double sphi, spsi;
+            angle_terms(m, pphi, ppsi, sphi, spsi);
+            double dps = 0.0;
+            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, phard, true, dps)) {
+                a.ps[(size_t)t * E + e] = plasticStrain + dps;
+                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            }
+        } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
+            /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
+            double Xp[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int nd = node_index(a, ei + kHexOff[kTetraToNode[t][q]][0], ej + kHexOff[kTetraToNode[t][q]][1],
+                                    ek + kHexOff[kTetraToNode[t][q]][2]);
+                Xp[q][0] = a.xp[nd];
+                Xp[q][1] = a.xp[a.N + nd];
+                Xp[q][2] = a.xp[2 * a.N + nd];
+            }
+            double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
+            double trace_strain = vol[t] / old_volume - 1.0;
+            double d0, d1, d2;
+            devstrain(sn, trace_strain, d0, d1, d2);
+            double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
+            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+            if (a.visc) a.visc[(size_t)t * E + e] = visc;
+            maxwell_update(m, st, sn, trace_strain, visc, dt);
+        } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
+            const double a2 = m.lambda;
+            const double sh2 = 2.0 * m.mu;
+            double trace_strain = sn[0] + sn[1] + sn[2];
+            st[0] += sh2 * sn[0] + a2 * (trace_strain);
+            st[1] += sh2 * sn[1] + a2 * (trace_strain);
+            st[2] += sh2 * sn[2] + a2 * (trace_strain);
+            st[3] += sh2 * sn[3];
+            st[4] += sh2 * sn[4];
+            st[5] += sh2 * sn[5];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) sig[t][c] = st[c];
+    }
+
+    /* pass 3: stress trace averaging (Stress.c:87-137) */
+    double pst = 0.0, tr[NTET];
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        tr[t] = (sig[t][0] + sig[t][1] + sig[t][2]) / 3.0;
+        pst += tr[t] * vol[t];
+    }
+    const double avgTrace = pst / elemVolume;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        sig[t][0] += -1.0 * tr[t] + avgTrace;
+        sig[t][1] += -1.0 * tr[t] + avgTrace;
+        sig[t][2] += -1.0 * tr[t] + avgTrace;
+#pragma unroll
+        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + e] = sig[t][c];
+    }
+}
+
+/* --------------------------------------------------------------------------
+ * k_rotated: stress with the pending rotation applied, for downloads
+ * ------------------------------------------------------------------------ */
+__global__ void __launch_bounds__(ELEM_BLOCK) k_rotated(StepArgs a, double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    const double dt_rot = a.scal->dt_rot;
+    double X[8][3], U[8][3];
+    load_hex(a, a.xc, ei, ej, ek, X);
+    load_hex(a, a.v, ei, ej, ek, U);
+    const size_t E = (size_t)a.E;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        double Xt[4][3], Ut[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                Xt[q][d] = X[kTetraToNode[t][q]][d];
+                Ut[q][d] = U[kTetraToNode[t][q]][d];
+            }
+        TetKin k;
+        tet_kinematics(Xt, Ut, k);
+        double st[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) st[c] = a.stress[((size_t)c * NTET + t) * E + e];
+        rotate_stress(st, k.w, dt_rot);
+#pragma unroll
+        for (int c = 0; c < 6; c++) out[((size_t)c * NTET + t) * E + e] = st[c];
+    }
+}
+
+/* volumes of the current geometry, for downloads */
+__global__ void k_volumes(StepArgs a, double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    double X[8][3];
+    load_hex(a, a.xc, ei, ej, ek, X);
+#pragma unroll
+    for (int t = 0; t < NTET; t++)
+        out[(size_t)t * a.E + e] = tet_volume(X[kTetraToNode[t][0]], X[kTetraToNode[t][1]], X[kTetraToNode[t][2]],
+                                              X[kTetraToNode[t][3]]);
+}
+
+/* --------------------------------------------------------------------------
+ * k_node: Snac_EP_LoopNodesMomentum
+ * ------------------------------------------------------------------------ */
+#define GETSIGN(val) ((val) < 0 ? -1.0 : (val) > 0 ? +1.0 : 0.0)
+
+__global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
+    __shared__ double sh[NODE_BLOCK / 64];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double vmag = -INFINITY;
+    if (n < a.N) {
+        const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+        const double dt = a.scal->dt;
+        const double c = a.scal->speedOfSound;
+        const size_t E = (size_t)a.E;
+        double im = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+#pragma unroll
+        for (int slot = 0; slot < 8; slot++) {
+            const int ei = i - 1 + (slot & 1), ej = j - 1 + ((slot >> 1) & 1), ek = k - 1 + ((slot >> 2) & 1);
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= a.EI || ej >= a.EJ || ek >= a.EK) continue;
+            const int e = ei + a.EI * (ej + a.EJ * ek);
+            const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+            const double inertialDensity = m.lam2mu / (c * c);
+            const double rho = m.rho_eff;
+            double X[8][3];
+            load_hex(a, a.xc, ei, ej, ek, X);
+#pragma unroll
+            for (int idx = 0; idx < 5; idx++) {
+                const int t = kNodeToTetra[slot][idx];
+                const int f = kNodeToSurface[slot][idx];
+                const double* p0 = X[kTetraToNode[t][0]];
+                const double* p1 = X[kTetraToNode[t][1]];
+                const double* p2 = X[kTetraToNode[t][2]];
+                const double* p3 = X[kTetraToNode[t][3]];
+                const double V = tet_volume(p0, p1, p2, p3);
+                double ax, ay, az;
+                face_cross(X[kTetraToNode[t][kFaceToNode[f][0]]], X[kTetraToNode[t][kFaceToNode[f][1]]],
+                           X[kTetraToNode[t][kFaceToNode[f][2]]], ax, ay, az);
+                const double sq = sqrt(ax * ax + ay * ay + az * az);
+                const double area = sq / 2.0;
+                const double n0 = ax / sq, n1 = ay / sq, n2 = az / sq;
+                const double s00 = a.stress[((size_t)0 * NTET + t) * E + e];
+                const double s11 = a.stress[((size_t)1 * NTET + t) * E + e];
+                const double s22 = a.stress[((size_t)2 * NTET + t) * E + e];
+                const double s01 = a.stress[((size_t)3 * NTET + t) * E + e];
+                const double s02 = a.stress[((size_t)4 * NTET + t) * E + e];
+                const double s12 = a.stress[((size_t)5 * NTET + t) * E + e];
+                if (a.dtType == SNH_DT_DYNAMIC) {
+                    im += 0.25 * inertialDensity * V;
+                } else if (a.dtType == SNH_DT_CONSTANT) {
+                    double area_sum = 0.0, tmp;
+                    tmp = (n0 * area) * (n0 * area);
+                    if (tmp > area_sum) area_sum = tmp;
+                    tmp = (n1 * area) * (n1 * area);
+                    if (tmp > area_sum) area_sum = tmp;
+                    tmp = (n2 * area) * (n2 * area);
+                    if (tmp > area_sum) area_sum = tmp;
+                    im += m.lam2mu * area_sum / (9.0 * V) * (dt * dt);
+                } else {
+                    im += 0.25 * rho * V;
+                }
+                f0 += (s00 * n0 + s01 * n1 + s02 * n2) * area / 6.0;
+                f1 += (s01 * n0 + s11 * n1 + s12 * n2) * area / 6.0;
+                f2 += (s02 * n0 + s12 * n1 + s22 * n2) * area / 6.0;
+                f1 -= a.gravity * rho * V / 8.0;
+            }
+        }
+        /* damping, velocity, BC, coordinates (UpdateNode.c:60-158) */
+        const size_t N = (size_t)a.N;
+        double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
+        f0 -= a.demf * GETSIGN(v0) * fabs(f0);
+        f1 -= a.demf * GETSIGN(v1) * fabs(f1);
+        f2 -= a.demf * GETSIGN(v2) * fabs(f2);
+        v0 += dt * f0 / im;
+        v1 += dt * f1 / im;
+        v2 += dt * f2 / im;
+        const uint32_t code = a.bc[n];
+        if (code) {
+            const DevBC& b = c_bc[code];
+            if (b.mask & 1u) v0 = b.val[0];
+            if (b.mask & 2u) v1 = b.val[1];
+            if (b.mask & 4u) v2 = b.val[2];
+            vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        }
+        a.v[n] = v0;
+        a.v[N + n] = v1;
+        a.v[2 * N + n] = v2;
+        a.xn[n] = a.xc[n] + dt * v0;
+        a.xn[N + n] = a.xc[N + n] + dt * v1;
+        a.xn[2 * N + n] = a.xc[2 * N + n] + dt * v2;
+    }
+    double bm = block_max(vmag, sh);
+    if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+}
+
+/* |v| over constrained nodes without an update (TimeStepZero's Sync) */
+__global__ void __launch_bounds__(NODE_BLOCK) k_vmax(StepArgs a) {
+    __shared__ double sh[NODE_BLOCK / 64];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double vmag = -INFINITY;
+    if (n < a.N && a.bc[n]) {
+        const size_t N = (size_t)a.N;
+        double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
+        vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    }
+    double bm = block_max(vmag, sh);
+    if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+}
+
+/* --------------------------------------------------------------------------
+ * k_geo: Snac_EP_LoopElementsMomentum's geometry and minLengthScale
+ * ------------------------------------------------------------------------ */
+__global__ void __launch_bounds__(GEO_BLOCK) k_geo(StepArgs a) {
+    __shared__ double sh[GEO_BLOCK / 64];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double emin = INFINITY;
+    if (e < a.E) {
+        const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+        double X[8][3];
+        load_hex(a, a.xc, ei, ej, ek, X);
+        emin = F_1EP27;
+#pragma unroll
+        for (int t = 0; t < NTET; t++) {
+            double Xt[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) Xt[q][d] = X[kTetraToNode[t][q]][d];
+            /* min over faces of V / (sqrt(ssq)/2): sqrt, halving and the
+               division are monotone, so the face with the largest sum of
+               squares gives the minimum exactly */
+            double ssq_max = 0.0, a0 = 0, b0 = 0, c0 = 0;
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                double fa, fb, fc;
+                face_cross(Xt[kFaceToNode[f][0]], Xt[kFaceToNode[f][1]], Xt[kFaceToNode[f][2]], fa, fb, fc);
+                if (f == 0) {
+                    a0 = fa;
+                    b0 = fb;
+                    c0 = fc;
+                }
+                double ssq = fa * fa + fb * fb + fc * fc;
+                ssq_max = ssq > ssq_max ? ssq : ssq_max;
+            }
+            const double V = tet_volume_from_face0(Xt[0], Xt[3], a0, b0, c0);
+            const double area = sqrt(ssq_max) / 2.0;
+            if (area) {
+                double ls = fabs(V / area);
+                if (emin > ls) emin = ls;
+            }
+        }
+    }
+    double bm = block_min(emin, sh);
+    if (threadIdx.x == 0) {
+        a.part_min[blockIdx.x] = bm;
+        if (blockIdx.x == 0) a.scal->dt_rot = a.scal->dt;
+    }
+}
+
+/* --------------------------------------------------------------------------
+ * k_fin: _Snac_Context_Sync
+ * ------------------------------------------------------------------------ */
+__global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvmax) {
+    __shared__ double sh[FIN_BLOCK / 64];
+    double mn = INFINITY, mx = -INFINITY;
+    for (int q = threadIdx.x; q < nmin; q += blockDim.x) mn = fmin(mn, a.part_min[q]);
+    for (int q = threadIdx.x; q < nvmax; q += blockDim.x) mx = fmax(mx, a.part_vmax[q]);
+    mn = block_min(mn, sh);
+    __syncthreads();
+    mx = block_max(mx, sh);
+    if (threadIdx.x == 0) {
+        double vmax = mx > 0.0 ? mx : 0.0;   /* vmax starts at 0 (Context.c:1218) */
+        if (vmax < F_3EM11) vmax = F_3EM11;
+        DevScalars* s = a.scal;
+        s->vmax = vmax;
+        s->minLen = mn;
+        if (a.dtType == SNH_DT_DYNAMIC) {
+            s->dt = mn * F_045 * a.strain_inert / vmax;
+            s->speedOfSound = mn * 0.5 / s->dt;
+        }
+    }
+}
+
+/* ==========================================================================
+ * host side
+ * ======================================================================== */
+static thread_local char g_err[512];
+
+static int set_err(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            return set_err(SNH_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                           __LINE__);                                                             \
+    } while (0)
+
+struct snh_context {
+    snh_config cfg;
+    std::vector<snh_material> mats;
+    std::vector<snh_velocity_bc> bcs;
+    int I, J, K, EI, EJ, EK, N, E;
+    hipStream_t stream;
+    bool own_stream;
+    /* device buffers */
+    double* x[2];
+    int cur;        /* index of the current coordinate buffer */
+    double* v;
+    double* stress;
+    double* ps;
+    double* visc;
+    uint8_t* mat;
+    double* refvisc;
+    double* cohoff;
+    uint8_t* bc;
+    double* part_min;
+    double* part_vmax;
+    DevScalars* scal;
+    double* tmp;    /* download staging [6][10][E] */
+    int n_geo_blocks, n_node_blocks;
+    unsigned rset;  /* rheologies present */
+    bool zero_done;
+    bool rotation_pending;
+    bool needs_plastic;
+    bool needs_visc;
+    uint32_t timeStep;
+};
+
+extern "C" int snh_abi_version(void) { return SNH_ABI_VERSION; }
+extern "C" const char* snh_last_error(void) { return g_err; }
+
+static void fill_dev_material(const snh_material& s, DevMaterial& d) {
+    memset(&d, 0, sizeof d);
+    d.rheology = s.rheology;
+    d.lambda = s.lambda;
+    d.mu = s.mu;
+    /* Stress.c:169-170 with avgTemp = 0 and beta = 0 */
+    d.rho_eff = s.phsDensity * (1.0 - s.alpha * (0.0 - s.reftemp));
+    d.lam2mu = s.lambda + 2.0 * s.mu;
+    d.nseg = s.nsegments;
+    for (unsigned i = 0; i <= s.nsegments && i < 9; i++) {
+        d.plstrain[i] = s.plstrain[i];
+        d.fa[i] = s.frictionAngle[i];
+        d.da[i] = s.dilationAngle[i];
+        d.coh[i] = s.cohesion[i];
+    }
+    d.ten_off = s.ten_off;
+    const double degrad = M_PI / 180.0;
+    d.sphi = sin(s.frictionAngle[0] * degrad);
+    d.spsi = sin(s.dilationAngle[0] * degrad);
+    d.tanphi = tan(s.frictionAngle[0] * degrad);
+    d.vis_min = s.vis_min;
+    d.vis_max = s.vis_max;
+    d.refvisc = s.refvisc;
+    d.refsrate = s.refsrate;
+    d.reftemp = s.reftemp;
+    d.activationE = s.activationE;
+    d.srexponent = s.srexponent;
+    d.linear_visc = (s.srexponent == 1.0 && s.activationE == 0.0);
+}
+
+static bool on_wall(const snh_context* c, int n, int wall) {
+    int i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
+    switch (wall) {
+    case SNH_WALL_FRONT: return c->K > 1 && k == 0;
+    case SNH_WALL_BACK: return c->K > 1 && k == c->K - 1;
+    case SNH_WALL_LEFT: return c->I > 1 && i == 0;
+    case SNH_WALL_RIGHT: return c->I > 1 && i == c->I - 1;
+    case SNH_WALL_TOP: return c->J > 1 && j == c->J - 1;
+    case SNH_WALL_BOTTOM: return c->J > 1 && j == 0;
+    case SNH_WALL_ALL: return true;
+    default: return false;
+    }
+}
+
+static StepArgs make_args(snh_context* c) {
+    StepArgs a;
+    a.xc = c->x[c->cur];
+    a.xp = c->x[c->cur ^ 1];
+    a.xn = c->x[c->cur ^ 1];
+    a.v = c->v;
+    a.stress = c->stress;
+    a.ps = c->ps;
+    a.visc = c->visc;
+    a.mat = c->mat;
+    a.refvisc = c->refvisc;
+    a.cohoff = c->cohoff;
+    a.bc = c->bc;
+    a.part_min = c->part_min;
+    a.part_vmax = c->part_vmax;
+    a.scal = c->scal;
+    a.I = c->I; a.J = c->J; a.K = c->K;
+    a.EI = c->EI; a.EJ = c->EJ; a.EK = c->EK;
+    a.N = c->N; a.E = c->E;
+    a.dtType = c->cfg.dtType;
+    a.gravity = c->cfg.gravity;
+    a.demf = c->cfg.demf;
+    a.strain_inert = c->cfg.strain_inert;
+    return a;
+}
+
+static void free_all(snh_context* c) {
+    void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
+                    c->bc, c->part_min, c->part_vmax, c->scal, c->tmp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
+    if (!cfg || !out) return set_err(SNH_EINVAL, "snh_context_create: null argument");
+    *out = nullptr;
+    if (cfg->meshSizeI < 2 || cfg->meshSizeJ < 2 || cfg->meshSizeK < 2)
+        return set_err(SNH_EINVAL, "mesh needs at least 2 nodes per axis");
+    if (cfg->nmaterials < 1 || cfg->nmaterials > SNH_MAX_MATERIALS || !cfg->materials)
+        return set_err(SNH_EINVAL, "need 1..%d materials", SNH_MAX_MATERIALS);
+    if (cfg->topo_kappa > 0)
+        return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0: surface diffusion is order dependent and not on the path");
+    if (cfg->dtType == SNH_DT_WAVE) return set_err(SNH_EUNSUPPORTED, "dtType wave is not implemented on the device");
+    if (cfg->dtType != SNH_DT_DYNAMIC && cfg->dtType != SNH_DT_CONSTANT) return set_err(SNH_EINVAL, "bad dtType");
+    if (cfg->precision != SNH_PRECISION_FP64)
+        return set_err(SNH_EUNSUPPORTED, "only SNH_PRECISION_FP64 is built in this library version");
+    if (cfg->nbcs && !cfg->bcs) return set_err(SNH_EINVAL, "nbcs > 0 with null bcs");
+    double vol = (double)cfg->meshSizeI * cfg->meshSizeJ * cfg->meshSizeK;
+    if (vol > 2.0e9) return set_err(SNH_EINVAL, "mesh too large for 32-bit node indices");
+    for (unsigned q = 0; q < cfg->nmaterials; q++) {
+        const snh_material& m = cfg->materials[q];
+        if (m.beta != 0.0)
+            return set_err(SNH_EUNSUPPORTED, "material %u: beta != 0 (pressure-dependent density) not on the device",
+                           q);
+        if (m.nsegments > SNH_MAX_SEGMENTS) return set_err(SNH_EINVAL, "material %u: too many segments", q);
+        for (unsigned s = 0; s <= m.nsegments; s++)
+            if ((m.rheology & (SNH_RHEOLOGY_PLASTIC | SNH_RHEOLOGY_VISCOPLASTIC)) && m.frictionAngle[s] < 0)
+                return set_err(SNH_EINVAL, "material %u: negative friction angle", q);
+    }
+
+    snh_context* c = new (std::nothrow) snh_context();
+    if (!c) return set_err(SNH_ENOMEM, "out of host memory");
+    c->cfg = *cfg;
+    c->mats.assign(cfg->materials, cfg->materials + cfg->nmaterials);
+    if (cfg->nbcs) c->bcs.assign(cfg->bcs, cfg->bcs + cfg->nbcs);
+    c->cfg.materials = c->mats.data();
+    c->cfg.bcs = c->bcs.empty() ? nullptr : c->bcs.data();
+    c->I = cfg->meshSizeI; c->J = cfg->meshSizeJ; c->K = cfg->meshSizeK;
+    c->EI = c->I - 1; c->EJ = c->J - 1; c->EK = c->K - 1;
+    c->N = c->I * c->J * c->K;
+    c->E = c->EI * c->EJ * c->EK;
+    c->rset = 0;
+    for (auto& m : c->mats) {
+        if (m.rheology & SNH_RHEOLOGY_PLASTIC) c->rset |= RS_PLASTIC;
+        else if (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC) c->rset |= RS_VISCOPLASTIC;
+        else if (m.rheology & SNH_RHEOLOGY_MAXWELL) c->rset |= RS_MAXWELL;
+        else if (m.rheology & SNH_RHEOLOGY_ELASTIC) c->rset |= RS_ELASTIC;
+        /* the law of a material is its highest rheology bit, matching one
+           rheology plugin per run */
+    }
+    for (auto& m : c->mats) {
+        if (m.rheology & SNH_RHEOLOGY_PLASTIC) m.rheology = SNH_RHEOLOGY_PLASTIC;
+        else if (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC) m.rheology = SNH_RHEOLOGY_VISCOPLASTIC;
+        else if (m.rheology & SNH_RHEOLOGY_MAXWELL) m.rheology = SNH_RHEOLOGY_MAXWELL;
+    }
+    c->needs_plastic = (c->rset & (RS_PLASTIC | RS_VISCOPLASTIC)) != 0;
+    c->needs_visc = false;
+    for (auto& m : c->mats)
+        if ((m.rheology & (SNH_RHEOLOGY_MAXWELL | SNH_RHEOLOGY_VISCOPLASTIC)) &&
+            !(m.srexponent == 1.0 && m.activationE == 0.0))
+            c->needs_visc = true;
+
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete c;
+        return set_err(SNH_EHIP, "hipStreamCreate: %s", hipGetErrorString(he));
+    }
+    c->own_stream = true;
+    const size_t N = c->N, E = c->E;
+    c->n_geo_blocks = (int)((E + GEO_BLOCK - 1) / GEO_BLOCK);
+    c->n_node_blocks = (int)((N + NODE_BLOCK - 1) / NODE_BLOCK);
+    bool ok = true;
+#define ALLOC(p, bytes) ok = ok && (hipMalloc((void**)&(p), (bytes)) == hipSuccess)
+    ALLOC(c->x[0], 3 * N * sizeof(double));
+    ALLOC(c->x[1], 3 * N * sizeof(double));
+    ALLOC(c->v, 3 * N * sizeof(double));
+    ALLOC(c->stress, 60 * E * sizeof(double));
+    if (c->needs_plastic) ALLOC(c->ps, 10 * E * sizeof(double));
+    if (c->needs_visc) ALLOC(c->visc, 10 * E * sizeof(double));
+    ALLOC(c->bc, N);
+    ALLOC(c->part_min, (size_t)c->n_geo_blocks * sizeof(double));
+    ALLOC(c->part_vmax, (size_t)c->n_node_blocks * sizeof(double));
+    ALLOC(c->scal, sizeof(DevScalars));
+#undef ALLOC
+    if (!ok) {
+        free_all(c);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return set_err(SNH_ENOMEM, "device allocation failed (%zu nodes, %zu elements)", N, E);
+    }
+
+    /* initial mesh (BlockGeometry.c:326-349), zero velocity + BCs, zero stress */
+    std::vector<double> hx(3 * N), hv(3 * N, 0.0);
+    const double mn[3] = {cfg->minX, cfg->minY, cfg->minZ}, mx[3] = {cfg->maxX, cfg->maxY, cfg->maxZ};
+    const int sz[3] = {c->I, c->J, c->K};
+    for (size_t n = 0; n < N; n++) {
+        int ijk[3] = {(int)(n % c->I), (int)((n / c->I) % c->J), (int)(n / ((size_t)c->I * c->J))};
+        for (int d = 0; d < 3; d++)
+            hx[d * N + n] = sz[d] > 1 ? mn[d] + (mx[d] - mn[d]) * ((double)ijk[d] / (double)(sz[d] - 1)) : mn[d];
+    }
+    /* BC codes: one per distinct (mask, values) combination (CompositeVC
+       semantics: union of the walls, last item wins per component) */
+    std::vector<uint8_t> hbc(N, 0);
+    std::vector<DevBC> table(1);
+    table[0].mask = 0;
+    for (size_t n = 0; n < N; n++) {
+        uint32_t mask = 0;
+        double val[3] = {0, 0, 0};
+        bool member = false;
+        for (auto& b : c->bcs) {
+            if (!on_wall(c, (int)n, b.wall)) continue;
+            member = true;
+            for (int d = 0; d < 3; d++)
+                if (b.mask & (1u << d)) {
+                    mask |= 1u << d;
+                    val[d] = b.value[d];
+                }
+        }
+        if (!member) continue;
+        size_t q;
+        for (q = 1; q < table.size(); q++)
+            if (table[q].mask == mask && !memcmp(table[q].val, val, sizeof val)) break;
+        if (q == table.size()) {
+            if (table.size() >= 256) {
+                free_all(c);
+                (void)hipStreamDestroy(c->stream);
+                delete c;
+                return set_err(SNH_EUNSUPPORTED, "more than 255 distinct velocity conditions");
+            }
+            DevBC b;
+            b.mask = mask;
+            memcpy(b.val, val, sizeof val);
+            table.push_back(b);
+        }
+        hbc[n] = (uint8_t)q;
+        for (int d = 0; d < 3; d++)
+            if (mask & (1u << d)) hv[d * N + n] = val[d];
+    }
+    std::vector<DevMaterial> dm(SNH_MAX_MATERIALS);
+    for (size_t q = 0; q < c->mats.size(); q++) fill_dev_material(c->mats[q], dm[q]);
+
+    int rc = SNH_OK;
+    do {
+        if (hipMemcpyToSymbol(HIP_SYMBOL(c_mat), dm.data(), sizeof(DevMaterial) * SNH_MAX_MATERIALS) != hipSuccess ||
+            hipMemcpyToSymbol(HIP_SYMBOL(c_bc), table.data(), sizeof(DevBC) * table.size()) != hipSuccess) {
+            rc = set_err(SNH_EHIP, "constant upload failed");
+            break;
+        }
+        if (hipMemcpy(c->x[0], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->x[1], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->v, hv.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->bc, hbc.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(c->stress, 0, 60 * E * sizeof(double)) != hipSuccess ||
+            (c->ps && hipMemset(c->ps, 0, 10 * E * sizeof(double)) != hipSuccess) ||
+            (c->visc && hipMemset(c->visc, 0, 10 * E * sizeof(double)) != hipSuccess)) {
+            rc = set_err(SNH_EHIP, "initial upload failed");
+            break;
+        }
+        DevScalars s0;
+        s0.dt = cfg->timeStep;
+        s0.dt_rot = cfg->timeStep;
+        s0.speedOfSound = 0.0;
+        s0.minLen = 0.0;
+        s0.vmax = 0.0;
+        if (hipMemcpy(c->scal, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = set_err(SNH_EHIP, "scalar upload failed");
+            break;
+        }
+    } while (0);
+    if (rc) {
+        free_all(c);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    c->cur = 0;
+    c->zero_done = false;
+    c->rotation_pending = false;
+    c->timeStep = 0;
+    *out = c;
+    return SNH_OK;
+}
+
+extern "C" int snh_context_destroy(snh_context* c) {
+    if (!c) return SNH_OK;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SNH_OK;
+}
+
+extern "C" int snh_set_stream(snh_context* c, void* stream) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return SNH_OK;
+}
+
+extern "C" int snh_set_element_material(snh_context* c, const uint8_t* mat, size_t nelem) {
+    if (!c || !mat) return set_err(SNH_EINVAL, "null argument");
+    if (nelem != (size_t)c->E) return set_err(SNH_EINVAL, "expected %d elements, got %zu", c->E, nelem);
+    for (size_t e = 0; e < nelem; e++)
+        if (mat[e] >= c->mats.size()) return set_err(SNH_EINVAL, "element %zu: material %u out of range", e, mat[e]);
+    if (!c->mat) HIPCHK(hipMalloc((void**)&c->mat, nelem));
+    HIPCHK(hipMemcpyAsync(c->mat, mat, nelem, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+extern "C" int snh_set_element_field(snh_context* c, int field, const double* values, size_t nelem) {
+    if (!c || !values) return set_err(SNH_EINVAL, "null argument");
+    if (nelem != (size_t)c->E) return set_err(SNH_EINVAL, "expected %d elements, got %zu", c->E, nelem);
+    double** dst = field == SNH_FIELD_REFVISC ? &c->refvisc : field == SNH_FIELD_COHESION_OFFSET ? &c->cohoff : nullptr;
+    if (!dst) return set_err(SNH_EINVAL, "unknown field %d", field);
+    if (!*dst) HIPCHK(hipMalloc((void**)dst, nelem * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(*dst, values, nelem * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
+    if (!c || !s) return set_err(SNH_EINVAL, "null argument");
+    const size_t N = c->N, E = c->E;
+    std::vector<double> buf;
+    if (s->coord) {
+        buf.resize(3 * N);
+        for (size_t n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) buf[d * N + n] = s->coord[3 * n + d];
+        HIPCHK(hipMemcpy(c->x[c->cur], buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->x[c->cur ^ 1], buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (s->velocity) {
+        buf.resize(3 * N);
+        for (size_t n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) buf[d * N + n] = s->velocity[3 * n + d];
+        HIPCHK(hipMemcpy(c->v, buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (s->stress) {
+        buf.resize(60 * E);
+        for (size_t e = 0; e < E; e++)
+            for (int t = 0; t < NTET; t++)
+                for (int q = 0; q < 6; q++) buf[((size_t)q * NTET + t) * E + e] = s->stress[(e * NTET + t) * 6 + q];
+        HIPCHK(hipMemcpy(c->stress, buf.data(), 60 * E * sizeof(double), hipMemcpyHostToDevice));
+        c->rotation_pending = false;
+    }
+    if (s->plastic_strain) {
+        if (!c->ps) return set_err(SNH_EINVAL, "no plastic rheology in this context");
+        buf.resize(10 * E);
+        for (size_t e = 0; e < E; e++)
+            for (int t = 0; t < NTET; t++) buf[(size_t)t * E + e] = s->plastic_strain[e * NTET + t];
+        HIPCHK(hipMemcpy(c->ps, buf.data(), 10 * E * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return SNH_OK;
+}
+
+static int launch_geo(snh_context* c) {
+    StepArgs a = make_args(c);
+    hipLaunchKernelGGL(k_geo, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return SNH_OK;
+}
+
+static int launch_fin(snh_context* c) {
+    StepArgs a = make_args(c);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, a, c->n_geo_blocks, c->n_node_blocks);
+    HIPCHK(hipGetLastError());
+    return SNH_OK;
+}
+
+static int launch_elem(snh_context* c) {
+    StepArgs a = make_args(c);
+    dim3 grid((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), block(ELEM_BLOCK);
+    switch (c->rset) {
+    case RS_ELASTIC: hipLaunchKernelGGL(k_elem<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
+    case RS_PLASTIC: hipLaunchKernelGGL(k_elem<RS_PLASTIC>, grid, block, 0, c->stream, a); break;
+    case RS_MAXWELL: hipLaunchKernelGGL(k_elem<RS_MAXWELL>, grid, block, 0, c->stream, a); break;
+    case RS_VISCOPLASTIC: hipLaunchKernelGGL(k_elem<RS_VISCOPLASTIC>, grid, block, 0, c->stream, a); break;
+    default:
+        hipLaunchKernelGGL(k_elem<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL | RS_VISCOPLASTIC>, grid, block, 0, c->stream,
+                           a);
+        break;
+    }
+    HIPCHK(hipGetLastError());
+    return SNH_OK;
+}
+
+static int launch_node(snh_context* c) {
+    StepArgs a = make_args(c);
+    hipLaunchKernelGGL(k_node, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    c->cur ^= 1;
+    return SNH_OK;
+}
+
+extern "C" int snh_hydrostatic_ic(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    const size_t N = c->N, E = c->E;
+    std::vector<double> x(3 * N), st(60 * E);
+    std::vector<uint8_t> mat;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(x.data(), c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(st.data(), c->stress, 60 * E * sizeof(double), hipMemcpyDeviceToHost));
+    if (c->mat) {
+        mat.resize(E);
+        HIPCHK(hipMemcpy(mat.data(), c->mat, E, hipMemcpyDeviceToHost));
+    }
+    const int elx = c->EI, ely = c->EJ, elz = c->EK;
+    auto yof = [&](int i, int j, int k) { return x[N + (size_t)(i + c->I * (j + c->J * k))]; };
+    for (int elK = 0; elK < elz; elK++)
+        for (int elI = 0; elI < elx; elI++) {
+            double rogh = 0.0;
+            for (int elJ = ely - 1; elJ > -1; elJ--) {
+                const size_t e = (size_t)elI + (size_t)elJ * elx + (size_t)elK * elx * ely;
+                const snh_material& m = c->mats[c->mat ? mat[e] : 0];
+                double densT = 0.0, dhE[4], dh, dPT, dP, P;
+                /* tet densities are still zero before TimeStepZero */
+                for (int t = 0; t < NTET; t++) densT += (m.phsDensity * (1.0 - m.alpha * (0.0 - m.reftemp))) / NTET;
+                /* element nodes 6-0, 2-4, 3-5, 7-1 (HexaEL.c:462-469 numbering) */
+                dhE[0] = yof(elI + 1, elJ + 1, elK + 1) - yof(elI, elJ, elK);
+                dhE[1] = yof(elI + 1, elJ + 1, elK) - yof(elI, elJ, elK + 1);
+                dhE[2] = yof(elI, elJ + 1, elK) - yof(elI + 1, elJ, elK + 1);
+                dhE[3] = yof(elI, elJ + 1, elK + 1) - yof(elI + 1, elJ, elK);
+                dh = 0.25 * (dhE[0] + dhE[1] + dhE[2] + dhE[3]);
+                dPT = densT * c->cfg.gravity * dh;
+                dP = dPT * (1.0 - m.beta * rogh) / (1.0 + m.beta / 2.0 * dPT);
+                P = rogh + 0.5 * dP;
+                P = -1.0 * (P + 0.0);
+                for (int t = 0; t < NTET; t++)
+                    for (int q = 0; q < 3; q++) st[((size_t)q * NTET + t) * E + e] = P;
+                rogh += dP;
+            }
+        }
+    HIPCHK(hipMemcpy(c->stress, st.data(), 60 * E * sizeof(double), hipMemcpyHostToDevice));
+    c->rotation_pending = false;
+    return SNH_OK;
+}
+
+extern "C" int snh_timestep_zero(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    int rc;
+    StepArgs a = make_args(c);
+    hipLaunchKernelGGL(k_vmax, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    if ((rc = launch_geo(c))) return rc;
+    if ((rc = launch_fin(c))) return rc;
+    c->zero_done = true;
+    c->rotation_pending = true;
+    c->timeStep = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+extern "C" int snh_calc_stresses(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_calc_stresses before snh_timestep_zero");
+    int rc = launch_elem(c);
+    if (rc) return rc;
+    c->rotation_pending = false;
+    c->timeStep++;
+    return SNH_OK;
+}
+
+extern "C" int snh_loop_nodes_momentum(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_nodes_momentum before snh_timestep_zero");
+    return launch_node(c);
+}
+
+extern "C" int snh_loop_elements_momentum(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_elements_momentum before snh_timestep_zero");
+    int rc = launch_geo(c);
+    if (rc) return rc;
+    c->rotation_pending = true;
+    return SNH_OK;
+}
+
+extern "C" int snh_sync(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    return launch_fin(c);
+}
+
+extern "C" int snh_step(snh_context* c, int nsteps) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_step before snh_timestep_zero");
+    if (nsteps < 0) return set_err(SNH_EINVAL, "negative step count");
+    for (int s = 0; s < nsteps; s++) {
+        int rc;
+        if ((rc = launch_elem(c))) return rc;
+        if ((rc = launch_node(c))) return rc;
+        if ((rc = launch_geo(c))) return rc;
+        if ((rc = launch_fin(c))) return rc;
+        c->timeStep++;
+    }
+    c->rotation_pending = c->rotation_pending || nsteps > 0;
+    return SNH_OK;
+}
+
+extern "C" int snh_synchronize(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+extern "C" int snh_get_scalars(snh_context* c, snh_scalars* out) {
+    if (!c || !out) return set_err(SNH_EINVAL, "null argument");
+    DevScalars s;
+    HIPCHK(hipMemcpyAsync(&s, c->scal, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    out->dt = s.dt;
+    out->speedOfSound = s.speedOfSound;
+    out->minLengthScale = s.minLen;
+    out->vmax = s.vmax;
+    out->timeStep = c->timeStep;
+    return SNH_OK;
+}
+
+extern "C" int snh_download_state(snh_context* c, snh_state* s) {
+    if (!c || !s) return set_err(SNH_EINVAL, "null argument");
+    const size_t N = c->N, E = c->E;
+    std::vector<double> buf;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (s->coord) {
+        buf.resize(3 * N);
+        HIPCHK(hipMemcpy(buf.data(), c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) s->coord[3 * n + d] = buf[d * N + n];
+    }
+    if (s->velocity) {
+        buf.resize(3 * N);
+        HIPCHK(hipMemcpy(buf.data(), c->v, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) s->velocity[3 * n + d] = buf[d * N + n];
+    }
+    if (s->stress || s->volume) {
+        if (!c->tmp) HIPCHK(hipMalloc((void**)&c->tmp, 60 * E * sizeof(double)));
+    }
+    if (s->stress) {
+        const double* src = c->stress;
+        if (c->rotation_pending) {
+            StepArgs a = make_args(c);
+            hipLaunchKernelGGL(k_rotated, dim3((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), dim3(ELEM_BLOCK), 0, c->stream,
+                               a, c->tmp);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+            src = c->tmp;
+        }
+        buf.resize(60 * E);
+        HIPCHK(hipMemcpy(buf.data(), src, 60 * E * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < E; e++)
+            for (int t = 0; t < NTET; t++)
+                for (int q = 0; q < 6; q++) s->stress[(e * NTET + t) * 6 + q] = buf[((size_t)q * NTET + t) * E + e];
+    }
+    if (s->volume) {
+        StepArgs a = make_args(c);
+        hipLaunchKernelGGL(k_volumes, dim3((c->E + 127) / 128), dim3(128), 0, c->stream, a, c->tmp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        buf.resize(10 * E);
+        HIPCHK(hipMemcpy(buf.data(), c->tmp, 10 * E * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < E; e++)
+            for (int t = 0; t < NTET; t++) s->volume[e * NTET + t] = buf[(size_t)t * E + e];
+    }
+    if (s->plastic_strain) {
+        if (c->ps) {
+            buf.resize(10 * E);
+            HIPCHK(hipMemcpy(buf.data(), c->ps, 10 * E * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t e = 0; e < E; e++)
+                for (int t = 0; t < NTET; t++) s->plastic_strain[e * NTET + t] = buf[(size_t)t * E + e];
+        } else {
+            memset(s->plastic_strain, 0, 10 * E * sizeof(double));
+        }
+    }
+    if (s->viscosity) {
+        if (c->visc) {
+            buf.resize(10 * E);
+            HIPCHK(hipMemcpy(buf.data(), c->visc, 10 * E * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t e = 0; e < E; e++)
+                for (int t = 0; t < NTET; t++) s->viscosity[e * NTET + t] = buf[(size_t)t * E + e];
+        } else {
+            /* linear law: the viscosity is the clamped reference viscosity */
+            std::vector<double> rv;
+            if (c->refvisc) {
+                rv.resize(E);
+                HIPCHK(hipMemcpy(rv.data(), c->refvisc, E * sizeof(double), hipMemcpyDeviceToHost));
+            }
+            std::vector<uint8_t> mat;
+            if (c->mat) {
+                mat.resize(E);
+                HIPCHK(hipMemcpy(mat.data(), c->mat, E, hipMemcpyDeviceToHost));
+            }
+            for (size_t e = 0; e < E; e++) {
+                const snh_material& m = c->mats[c->mat ? mat[e] : 0];
+                double vis = 0.0;
+                if ((m.rheology & (SNH_RHEOLOGY_MAXWELL | SNH_RHEOLOGY_VISCOPLASTIC)) && c->timeStep > 0) {
+                    vis = c->refvisc ? rv[e] : m.refvisc;
+                    if (vis < m.vis_min) vis = m.vis_min;
+                    if (vis > m.vis_max) vis = m.vis_max;
+                }
+                for (int t = 0; t < NTET; t++) s->viscosity[e * NTET + t] = vis;
+            }
+        }
+    }
+    return SNH_OK;
+}

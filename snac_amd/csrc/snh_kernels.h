@@ -1,0 +1,184 @@
+/*
+ * snh_kernels.h -- device-side data structures and per-tetrahedron arithmetic
+ * of the explicit step for gfx950 (FP64 path).
+ *
+ * Layout in HBM (structure of arrays, one thread per element or node):
+ *   coordinates   [3][N]      two buffers (current / previous step)
+ *   velocity      [3][N]
+ *   stress        [6][10][E]  components 00,11,22,01,02,12 of each tetrahedron
+ *   plastic strain[10][E]
+ * Every expression below keeps the operand order of the reference line it
+ * cites; the file is compiled with -ffp-contract=off.
+ */
+#ifndef SNH_KERNELS_H
+#define SNH_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "snh_math.h"
+
+namespace snh {
+
+constexpr int NTET = 10;
+
+/* Snac/libSnac/src/TetrahedraTables.c:42-84 (the reference's 2 x 5-tet
+   decomposition of the hex; identical tables keep the stored tet stresses
+   meaning the same). */
+static constexpr unsigned char kTetraToNode[NTET][4] = {
+    {7, 2, 3, 0}, {0, 5, 4, 7}, {2, 5, 1, 0}, {5, 2, 6, 7}, {2, 5, 0, 7},
+    {3, 4, 7, 6}, {4, 3, 0, 1}, {6, 1, 2, 3}, {1, 6, 5, 4}, {1, 3, 6, 4}};
+static constexpr unsigned char kFaceToNode[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {3, 1, 0}};
+static constexpr unsigned char kNodeToTetra[8][5] = {
+    {3, 5, 7, 8, 9}, {0, 1, 3, 4, 5}, {1, 2, 3, 4, 8}, {1, 5, 6, 8, 9},
+    {0, 2, 3, 4, 7}, {0, 5, 6, 7, 9}, {2, 6, 7, 8, 9}, {0, 1, 2, 4, 6}};
+static constexpr unsigned char kNodeToSurface[8][5] = {
+    {3, 0, 1, 2, 3}, {1, 0, 0, 0, 3}, {2, 2, 1, 2, 3}, {3, 2, 1, 0, 0},
+    {2, 1, 2, 1, 3}, {3, 1, 2, 0, 2}, {3, 0, 2, 1, 1}, {0, 1, 0, 3, 3}};
+/* hex-local node -> (di, dj, dk) offset (HexaEL.c:462-469) */
+static constexpr unsigned char kHexOff[8][3] = {
+    {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+/* double of the reference's float literals */
+constexpr double F_045 = (double)0.45f;    /* Context.c:1251 */
+constexpr double F_3EM11 = (double)3.0e-11f; /* Context.c:1219 */
+constexpr double F_1EP27 = (double)1.0e+27f; /* UpdateElement.c:50 */
+
+struct DevMaterial {
+    uint32_t rheology;
+    double lambda, mu;
+    double rho_eff;          /* density after Stress.c:169-170 with beta = 0 */
+    double lam2mu;           /* lambda + 2.0f*mu (Force.c:96) */
+    /* plasticity */
+    uint32_t nseg;
+    double plstrain[9], fa[9], da[9], coh[9];
+    double ten_off;
+    double sphi, spsi, tanphi;  /* host libm sin/sin/tan of fa[0], da[0] */
+    /* viscosity */
+    double vis_min, vis_max, refvisc, refsrate, reftemp, activationE, srexponent;
+    int linear_visc;         /* srexponent == 1 and activationE == 0 */
+};
+
+struct DevScalars {
+    double dt;             /* dt of the step being run                    */
+    double dt_rot;         /* dt of the Jaumann rotation still to apply   */
+    double speedOfSound;
+    double minLen;
+    double vmax;
+};
+
+struct DevBC {
+    uint32_t mask;
+    double val[3];
+};
+
+struct StepArgs {
+    const double* __restrict__ xc;  /* coordinates of this step [3][N]     */
+    const double* __restrict__ xp;  /* coordinates of the step before      */
+    double* __restrict__ xn;        /* coordinates written by the node pass */
+    double* __restrict__ v;         /* velocity [3][N]                     */
+    double* __restrict__ stress;    /* [6][10][E]                          */
+    double* __restrict__ ps;        /* [10][E] or null                     */
+    double* __restrict__ visc;      /* [10][E] or null                     */
+    const uint8_t* __restrict__ mat;      /* [E] or null (material 0)      */
+    const double* __restrict__ refvisc;   /* [E] or null                   */
+    const double* __restrict__ cohoff;    /* [E] or null                   */
+    const uint8_t* __restrict__ bc;       /* [N] BC code, 0 = none         */
+    double* __restrict__ part_min;
+    double* __restrict__ part_vmax;
+    DevScalars* __restrict__ scal;
+    int32_t I, J, K;        /* node counts   */
+    int32_t EI, EJ, EK;     /* element counts */
+    int32_t N, E;
+    int32_t dtType;
+    double gravity, demf, strain_inert;
+};
+
+/* kinematics of one tetrahedron from its 4 node coordinates and velocities,
+   in TetraToNode order */
+struct TetKin {
+    double V;
+    double sr[6];   /* raw strain rate 00,11,22,01,02,12 (StrainRate.c:65-184) */
+    double w[3];    /* rotation 01,02,12 (UpdateElement.c:145-219)            */
+};
+
+/* cross product of face (p1,p2,p3) as Tetrahedra_SurfaceArea/Normal compute it */
+SNH_HD void face_cross(const double* c1, const double* c2, const double* c3, double& a, double& b, double& c) {
+    double x2 = c2[0] - c1[0], y2 = c2[1] - c1[1], z2 = c2[2] - c1[2];
+    double x3 = c3[0] - c1[0], y3 = c3[1] - c1[1], z3 = c3[2] - c1[2];
+    a = y2 * z3 - z2 * y3;
+    b = z2 * x3 - x2 * z3;
+    c = x2 * y3 - y2 * x3;
+}
+
+/* Tetrahedra_Volume (Tetrahedra.c:44-67).  Its cross product is the one of
+   face 0 (nodes 0,1,2): x3*y2 == y2*x3 exactly. */
+SNH_HD double tet_volume_from_face0(const double* c1, const double* c4, double a0, double b0, double c0) {
+    double x4 = c4[0] - c1[0], y4 = c4[1] - c1[1], z4 = c4[2] - c1[2];
+    double v = (x4 * a0 + y4 * b0 + z4 * c0) / 6.0;
+    return fabs(v);
+}
+
+SNH_HD double tet_volume(const double* c1, const double* c2, const double* c3, const double* c4) {
+    double a, b, c;
+    face_cross(c1, c2, c3, a, b, c);
+    return tet_volume_from_face0(c1, c4, a, b, c);
+}
+
+/* geometry + velocity-gradient terms of one tet (UpdateElement.c:84-219,
+   StrainRate.c:65-184): node k pairs with face (k+1)%4 */
+SNH_HD void tet_kinematics(const double X[4][3], const double U[4][3], TetKin& o) {
+    double n[4][3], A[4];
+    double a0 = 0, b0 = 0, c0 = 0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        double a, b, c;
+        face_cross(X[kFaceToNode[f][0]], X[kFaceToNode[f][1]], X[kFaceToNode[f][2]], a, b, c);
+        if (f == 0) {
+            a0 = a;
+            b0 = b;
+            c0 = c;
+        }
+        double s = sqrt(a * a + b * b + c * c);
+        A[f] = s / 2.0;
+        n[f][0] = a / s;
+        n[f][1] = b / s;
+        n[f][2] = c / s;
+    }
+    const double V = tet_volume_from_face0(X[0], X[3], a0, b0, c0);
+    o.V = V;
+#define T_(k, p, q) (U[k][p] * n[((k) + 1) & 3][q] * A[((k) + 1) & 3])
+    /* products shared between strain rate and rotation */
+    const double s00 = T_(0, 0, 0) + T_(1, 0, 0) + T_(2, 0, 0) + T_(3, 0, 0);
+    const double s11 = T_(0, 1, 1) + T_(1, 1, 1) + T_(2, 1, 1) + T_(3, 1, 1);
+    const double s22 = T_(0, 2, 2) + T_(1, 2, 2) + T_(2, 2, 2) + T_(3, 2, 2);
+    const double s01 = T_(0, 0, 1) + T_(1, 0, 1) + T_(2, 0, 1) + T_(3, 0, 1);
+    const double s02 = T_(0, 0, 2) + T_(1, 0, 2) + T_(2, 0, 2) + T_(3, 0, 2);
+    const double s12 = T_(0, 1, 2) + T_(1, 1, 2) + T_(2, 1, 2) + T_(3, 1, 2);
+    o.sr[0] = -1.0 * s00 / 3.0 / V;
+    o.sr[1] = -1.0 * s11 / 3.0 / V;
+    o.sr[2] = -1.0 * s22 / 3.0 / V;
+    o.sr[3] = -0.5 * (s01 + T_(0, 1, 0) + T_(1, 1, 0) + T_(2, 1, 0) + T_(3, 1, 0)) / 3.0 / V;
+    o.sr[4] = -0.5 * (s02 + T_(0, 2, 0) + T_(1, 2, 0) + T_(2, 2, 0) + T_(3, 2, 0)) / 3.0 / V;
+    o.sr[5] = -0.5 * (s12 + T_(0, 2, 1) + T_(1, 2, 1) + T_(2, 2, 1) + T_(3, 2, 1)) / 3.0 / V;
+    const double hv = -(0.5 / V);
+    o.w[0] = hv * (s01 - T_(0, 1, 0) - T_(1, 1, 0) - T_(2, 1, 0) - T_(3, 1, 0)) / 3.0;
+    o.w[1] = hv * (s02 - T_(0, 2, 0) - T_(1, 2, 0) - T_(2, 2, 0) - T_(3, 2, 0)) / 3.0;
+    o.w[2] = hv * (s12 - T_(0, 2, 1) - T_(1, 2, 1) - T_(2, 2, 1) - T_(3, 2, 1)) / 3.0;
+#undef T_
+}
+
+/* in-place, in-order Jaumann rotation of a tet stress (UpdateElement.c:245-265) */
+SNH_HD void rotate_stress(double st[6], const double w[3], double dt) {
+    const double w01 = w[0], w02 = w[1], w12 = w[2];
+    st[0] += (2.0 * w01 * st[3] + 2.0 * w02 * st[4]) * dt;
+    st[1] += (2.0 * (-1.0 * w01) * st[3] + 2.0 * w12 * st[5]) * dt;
+    st[2] += (2.0 * (-1.0 * w02) * st[4] + 2.0 * (-1.0 * w12) * st[5]) * dt;
+    st[3] += (w01 * (st[1] - st[0])) * dt + (w02 * st[5] + w12 * st[4]) * dt;
+    st[4] += (w02 * (st[2] - st[0])) * dt + (w01 * st[5] - w12 * st[3]) * dt;
+    st[5] += (w12 * (st[2] - st[1])) * dt + ((-1.0 * w01) * st[4] - w02 * st[3]) * dt;
+}
+
+}  // namespace snh
+
+#endif

@@ -1,0 +1,18 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+    config.addinivalue_line("markers", "slow: larger CPU cases")
+
+
+def pytest_runtest_logstart(nodeid, location):
+    # name the case before any GPU work starts, so a hard failure is attributable
+    print(f"\n[start] {nodeid}", flush=True)
