@@ -1,0 +1,168 @@
+"""Throughput of SNAC's explicit time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one explicit time step of the whole subdomain (strain rate, stress,
+force gather, node update, geometry, dt) on synthetic input of the named
+workload.  Default workload: Mohr-Coulomb extension, 256 x 64 x 128 hex
+elements (x, y vertical, z) over 400 x 100 x 200 km, FP64.
+
+Prints one JSON line: element-steps/s over exactly K timed steps (after W
+warm-up steps, device synchronised on both sides), the roofline of the
+dominant kernel from HIP events recorded live on the library's stream, and the
+CPU oracle timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def make_problem(name):
+    from snac_amd import problem as P
+    if name == "mohr_coulomb_extension":
+        return P.mohr_coulomb_extension()
+    if name == "elastic_box_compression":
+        return P.elastic_box_compression()
+    if name == "maxwell_viscoelastic":
+        return P.maxwell_viscoelastic()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(problem, steps, threads):
+    """The CPU oracle (C restatement of the reference step, gcc -O2
+    -ffp-contract=off, OpenMP over the element and node loops) timed on this
+    host: TimeStepZero, one warm-up step, then `steps` timed steps."""
+    from oracle.oracle import Oracle
+    o = Oracle(problem, threads=threads)
+    if problem.hydrostatic:
+        o.hydrostatic_ic()
+    o.timestep_zero()
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    o.close()
+    return problem.n_elements * steps / (t1 - t0), t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="mohr_coulomb_extension")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores given to this job")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    from snac_amd import Context, _abi, metrics
+    problem = make_problem(args.workload)
+    E = problem.n_elements
+
+    ctx = Context(problem).Construct().Build().Initialise()
+    ctx.step(args.warmup)
+    ctx.synchronize()
+
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times()
+    ctx.profile(False)
+    sc = ctx.scalars()
+    ctx.Destroy()
+
+    wall = t1 - t0
+    value = E * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+
+    # dominant kernel and its roofline
+    per = {k: kt[k + "_ms"] / max(1, kt[k + "_n"]) for k in ("elem", "node", "geo", "fin")}
+    dom = max(("elem", "node", "geo"), key=lambda k: per[k])
+    dom_bytes = {"elem": metrics.elem_kernel_bytes_per_element(problem) * E,
+                 "node": metrics.node_kernel_bytes_per_element(problem) * E,
+                 "geo": 24.0 * problem.n_nodes}[dom]
+    achieved = dom_bytes / (per[dom] * 1e-3) / 1e9
+    lib = _abi.load_library()
+    bw = C.c_double(0.0)
+    rc = lib.snh_measure_copy_bandwidth(1 << 30, 20, C.byref(bw))
+    measured_peak = bw.value if rc == 0 else None
+    step_bytes = metrics.step_bytes_per_element(problem)
+    kernel_sum_ms = sum(kt[k + "_ms"] for k in ("elem", "node", "geo", "fin")) / args.steps
+
+    out = {
+        "metric": "element-steps/s",
+        "value": value,
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": problem.name, "elements": list(problem.elements),
+                   "nodes": list(problem.nodes), "precision": "fp64"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": {"elem": "k_elem", "node": "k_node", "geo": "k_geo"}[dom],
+            "achieved": achieved,
+            "peak": metrics.HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / metrics.HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": dom_bytes,
+            "avg_launch_ms": per[dom],
+            "measured_copy_peak": measured_peak,
+        },
+        "step_roofline": {
+            "bytes_per_element_step": step_bytes,
+            "achieved": value * step_bytes / 1e9,
+            "frac": value * step_bytes / 1e9 / metrics.HBM_PEAK_GBPS,
+            "kernel_ms_per_step": {k: per[k] for k in per},
+            "kernel_sum_ms_per_step": kernel_sum_ms,
+        },
+        "final_dt": sc["dt"],
+    }
+
+    if not args.no_cpu and args.cpu_steps > 0:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        cval, csec = cpu_baseline(problem, args.cpu_steps, threads)
+        out["cpu_baseline"] = {
+            "value": cval, "unit": "element-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{problem.name} {problem.elements} full mesh, {args.cpu_steps} timed steps after "
+                      f"TimeStepZero + 1 warm-up step ({csec:.1f} s), CPU oracle gcc -O2 -ffp-contract=off "
+                      f"OpenMP on {cpu_model()}",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -218,6 +218,25 @@ int snh_step(snh_context* ctx, int nsteps);
 int snh_get_scalars(snh_context* ctx, snh_scalars* out);
 int snh_synchronize(snh_context* ctx);
 
+/* Per-kernel durations measured with HIP events recorded on the context's
+ * stream around every launch while profiling is on (no counterpart in the
+ * reference; its TAU/CallGraph instrumentation, README_PROFILING and
+ * Snac/libSnac/src/EntryPoint.c:162-173, plays this role on the CPU).
+ * snh_profile(ctx, 1) clears the accumulators and starts recording;
+ * snh_get_kernel_times waits for the stream and returns totals in milliseconds. */
+typedef struct snh_kernel_times {
+    double   elem_ms, node_ms, geo_ms, fin_ms;
+    uint32_t elem_n, node_n, geo_n, fin_n;
+} snh_kernel_times;
+int snh_profile(snh_context* ctx, int enable);
+
+/* Device-to-device copy bandwidth of the current GPU (GB/s, read + write
+ * bytes), from a grid-stride copy of nbytes timed with HIP events over iters
+ * launches: the measured HBM peak reported beside the 8 TB/s specification.
+ * Utility with no counterpart in the reference. */
+int snh_measure_copy_bandwidth(size_t nbytes, int iters, double* gbps);
+int snh_get_kernel_times(snh_context* ctx, snh_kernel_times* out);
+
 #ifdef __cplusplus
 }
 #endif

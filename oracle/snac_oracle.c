@@ -287,6 +287,7 @@ orc_context* orc_create(const snh_config* cfg) {
     c->cfg = *cfg;
     c->nmat = cfg->nmaterials;
     memcpy(c->mat, cfg->materials, sizeof(snh_material) * cfg->nmaterials);
+    c->cfg.bcs = NULL;   /* owned copy below; never the caller's array */
     if (cfg->nbcs) {
         snh_velocity_bc* b = (snh_velocity_bc*)malloc(sizeof(snh_velocity_bc) * cfg->nbcs);
         memcpy(b, cfg->bcs, sizeof(snh_velocity_bc) * cfg->nbcs);

@@ -115,6 +115,11 @@ class Scalars(C.Structure):
     ]
 
 
+class KernelTimes(C.Structure):
+    _fields_ = [("elem_ms", C.c_double), ("node_ms", C.c_double), ("geo_ms", C.c_double), ("fin_ms", C.c_double),
+                ("elem_n", C.c_uint32), ("node_n", C.c_uint32), ("geo_n", C.c_uint32), ("fin_n", C.c_uint32)]
+
+
 # Every function the header declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -137,6 +142,9 @@ SIGNATURES = {
     "snh_step": (_I, [_P, _I]),
     "snh_get_scalars": (_I, [_P, C.POINTER(Scalars)]),
     "snh_synchronize": (_I, [_P]),
+    "snh_profile": (_I, [_P, _I]),
+    "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
+    "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))

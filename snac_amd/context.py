@@ -105,6 +105,15 @@ class Context:
     def synchronize(self):
         self._check(self.lib.snh_synchronize(self.h))
 
+    def profile(self, enable: bool = True):
+        self._check(self.lib.snh_profile(self.h, int(enable)))
+
+    def kernel_times(self) -> Dict[str, float]:
+        k = _abi.KernelTimes()
+        self._check(self.lib.snh_get_kernel_times(self.h, C.byref(k)))
+        return {"elem_ms": k.elem_ms, "node_ms": k.node_ms, "geo_ms": k.geo_ms, "fin_ms": k.fin_ms,
+                "elem_n": k.elem_n, "node_n": k.node_n, "geo_n": k.geo_n, "fin_n": k.fin_n}
+
     # -- state ----------------------------------------------------------------
     def scalars(self) -> Dict[str, float]:
         s = _abi.Scalars()

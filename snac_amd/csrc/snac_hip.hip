@@ -457,203 +457,259 @@ __device__ __forceinline__ void devstrain(const double sn[6], double trace_strai
 }
 
 /* --------------------------------------------------------------------------
- * k_elem: Snac_EP_CalcStresses with the pending rotation
+ * k_elem: Snac_EP_CalcStresses with the pending rotation.
+ *
+ * One lane per tetrahedron: a wave holds 6 elements x 10 tets (lanes 60..63
+ * idle).  Element-level sums (volume, strain-rate and stress trace averages)
+ * are formed in tet order from LDS, so they round exactly as the reference's
+ * sequential loops.
  * ------------------------------------------------------------------------ */
 enum { RS_ELASTIC = 1, RS_PLASTIC = 2, RS_MAXWELL = 4, RS_VISCOPLASTIC = 8 };
+#define EPW 6   /* elements per wave */
+
+/* TetraToNode packed 3 bits per node, 12 bits per tet (TetrahedraTables.c:42-53) */
+__host__ __device__ __forceinline__ int tet_node(int t, int q) {
+    /* t in 0..9, q in 0..3: decode from two 64-bit words */
+    constexpr unsigned long long P0 =
+        (0x7ull << 0) | (0x2ull << 3) | (0x3ull << 6) | (0x0ull << 9) |          /* tet 0: 7 2 3 0 */
+        (0x0ull << 12) | (0x5ull << 15) | (0x4ull << 18) | (0x7ull << 21) |      /* tet 1: 0 5 4 7 */
+        (0x2ull << 24) | (0x5ull << 27) | (0x1ull << 30) | (0x0ull << 33) |      /* tet 2: 2 5 1 0 */
+        (0x5ull << 36) | (0x2ull << 39) | (0x6ull << 42) | (0x7ull << 45) |      /* tet 3: 5 2 6 7 */
+        (0x2ull << 48) | (0x5ull << 51) | (0x0ull << 54) | (0x7ull << 57);       /* tet 4: 2 5 0 7 */
+    constexpr unsigned long long P1 =
+        (0x3ull << 0) | (0x4ull << 3) | (0x7ull << 6) | (0x6ull << 9) |          /* tet 5: 3 4 7 6 */
+        (0x4ull << 12) | (0x3ull << 15) | (0x0ull << 18) | (0x1ull << 21) |      /* tet 6: 4 3 0 1 */
+        (0x6ull << 24) | (0x1ull << 27) | (0x2ull << 30) | (0x3ull << 33) |      /* tet 7: 6 1 2 3 */
+        (0x1ull << 36) | (0x6ull << 39) | (0x5ull << 42) | (0x4ull << 45) |      /* tet 8: 1 6 5 4 */
+        (0x1ull << 48) | (0x3ull << 51) | (0x6ull << 54) | (0x4ull << 57);       /* tet 9: 1 3 6 4 */
+    const unsigned long long w = t < 5 ? P0 : P1;
+    const int sh = (t < 5 ? t : t - 5) * 12 + q * 3;
+    return (int)((w >> sh) & 7ull);
+}
+
+/* hex-local node -> node index (HexaEL.c:462-469 numbering) */
+__device__ __forceinline__ int hex_node_index(const StepArgs& a, int ei, int ej, int ek, int q) {
+    const int di = (q ^ (q >> 1)) & 1, dj = (q >> 1) & 1, dk = q >> 2;
+    return (ei + di) + a.I * ((ej + dj) + a.J * (ek + dk));
+}
+
+/* softened parameters carried along the tets of an element: the reference
+   declares them once per element (plastic_BI/Constitutive.c:78-81,
+   viscoplastic_BI/Constitutive.c:115-119) */
+struct Soft {
+    double phi, psi, coh, hard;
+};
 
 template <int RSET>
-__global__ void __launch_bounds__(ELEM_BLOCK) k_elem(StepArgs a) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.E) return;
-    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+__global__ void __launch_bounds__(64) k_elem(StepArgs a) {
+    __shared__ double sh_a[EPW][NTET];
+    __shared__ double sh_b[EPW][NTET];
+    __shared__ double sh_sp[EPW][NTET][5];   /* matched, phi, psi, coh, hard */
+    __shared__ double sh_ps[EPW][NTET];
+    const int lane = threadIdx.x;
+    /* lanes 60..63 carry no tet: they mirror tet 0 of element slot 0 but
+       never write LDS or memory */
+    const bool in_wave = lane < EPW * NTET;
+    const int le = in_wave ? lane / NTET : 0;
+    const int t = in_wave ? lane - le * NTET : 0;
+    const int e = blockIdx.x * EPW + le;
+    const bool active = in_wave && e < a.E;
+    const size_t E = (size_t)a.E;
+    const int ec = active ? e : 0;
+    const int ei = ec % a.EI, ej = (ec / a.EI) % a.EJ, ek = ec / (a.EI * a.EJ);
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
-    const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+    const DevMaterial& m = c_mat[a.mat ? a.mat[ec] : 0];
 
-    double X[8][3], U[8][3];
-    load_hex(a, a.xc, ei, ej, ek, X);
-    load_hex(a, a.v, ei, ej, ek, U);
-
-    /* pass 1: kinematics of the 10 tets */
-    double sr[NTET][6], w[NTET][3], vol[NTET];
-    double evol = 0.0;
+    /* kinematics of this lane's tet (UpdateElement.c:84-219, StrainRate.c:65-184) */
+    int nd[4];
+    double Xt[4][3], Ut[4][3];
 #pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        double Xt[4][3], Ut[4][3];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                Xt[q][d] = X[kTetraToNode[t][q]][d];
-                Ut[q][d] = U[kTetraToNode[t][q]][d];
-            }
-        TetKin k;
-        tet_kinematics(Xt, Ut, k);
-        vol[t] = k.V;
-        evol += k.V;
-#pragma unroll
-        for (int c = 0; c < 6; c++) sr[t][c] = k.sr[c];
-#pragma unroll
-        for (int c = 0; c < 3; c++) w[t][c] = k.w[c];
+    for (int q = 0; q < 4; q++) {
+        nd[q] = hex_node_index(a, ei, ej, ek, tet_node(t, q));
+        Xt[q][0] = a.xc[nd[q]];
+        Xt[q][1] = a.xc[a.N + nd[q]];
+        Xt[q][2] = a.xc[2 * a.N + nd[q]];
+        Ut[q][0] = a.v[nd[q]];
+        Ut[q][1] = a.v[a.N + nd[q]];
+        Ut[q][2] = a.v[2 * a.N + nd[q]];
     }
-    evol *= 0.5;                        /* UpdateElement.c:267-268 */
+    TetKin k;
+    tet_kinematics(Xt, Ut, k);
+    const double trace = (k.sr[0] + k.sr[1] + k.sr[2]) / 3.0;
+    if (in_wave) {
+        sh_a[le][t] = k.V;
+        sh_b[le][t] = trace * k.V;
+    }
+    __syncthreads();
+    double evol = 0.0, partial = 0.0;
+#pragma unroll
+    for (int q = 0; q < NTET; q++) {
+        evol += sh_a[le][q];        /* UpdateElement.c:141 */
+        partial += sh_b[le][q];     /* StrainRate.c:281 */
+    }
+    evol *= 0.5;                          /* UpdateElement.c:267-268 */
     const double elemVolume = 2.0 * evol; /* StrainRate.c:57, Stress.c:68 */
-    double partial = 0.0;
-    double trace[NTET];
-#pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        trace[t] = (sr[t][0] + sr[t][1] + sr[t][2]) / 3.0;
-        partial += trace[t] * vol[t];
-    }
     const double srcorr = partial / elemVolume;
 
-    /* pass 2: rotation, strain, constitutive */
-    double sig[NTET][6];
-    const size_t E = (size_t)a.E;
-    /* plastic_BI / viscoplastic_BI keep their softened parameters across the
-       tets of an element */
-    double pcoh = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0e6 : 0.0;
-    double pphi = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0 : 0.0;
-    double ppsi = 0.0, phard = 0.0, ptc = 0.0;
-    const double cohoff = a.cohoff ? a.cohoff[e] : 0.0;
+    /* rotation of the stored stress, strain (Stress.c:71-82) */
+    double st[6];
 #pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        double st[6];
+    for (int c = 0; c < 6; c++) st[c] = active ? a.stress[((size_t)c * NTET + t) * E + ec] : 0.0;
+    rotate_stress(st, k.w, dt_rot);
+    double sn[6];
 #pragma unroll
-        for (int c = 0; c < 6; c++) st[c] = a.stress[((size_t)c * NTET + t) * E + e];
-        rotate_stress(st, w[t], dt_rot);
-        double s6[6];
+    for (int c = 0; c < 3; c++) sn[c] = (k.sr[c] + (-1.0 * trace + srcorr)) * dt;
 #pragma unroll
-        for (int c = 0; c < 3; c++) s6[c] = sr[t][c] + (-1.0 * trace[t] + srcorr);
-#pragma unroll
-        for (int c = 3; c < 6; c++) s6[c] = sr[t][c];
-        double sn[6];
-#pragma unroll
-        for (int c = 0; c < 6; c++) sn[c] = s6[c] * dt;
+    for (int c = 3; c < 6; c++) sn[c] = k.sr[c] * dt;
 
-        if ((RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC)) {
-            double plasticStrain = a.ps[(size_t)t * E + e];
-            double trace_strain = sn[0] + sn[1] + sn[2];
-            double S[3][3];
-            st[0] += (2.0 * m.mu) * sn[0] + m.lambda * (trace_strain);
-            st[1] += (2.0 * m.mu) * sn[1] + m.lambda * (trace_strain);
-            st[2] += (2.0 * m.mu) * sn[2] + m.lambda * (trace_strain);
-            st[3] += (2.0 * m.mu) * sn[3];
-            st[4] += (2.0 * m.mu) * sn[4];
-            st[5] += (2.0 * m.mu) * sn[5];
-            S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
-            S[0][1] = S[1][0] = st[3];
-            S[0][2] = S[2][0] = st[4];
-            S[1][2] = S[2][1] = st[5];
-            double d3[3], cn[3][3];
-            principal_stresses(S, d3, cn);
-            soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
-            double coh = a.cohoff ? pcoh + cohoff : pcoh;
-            if (pphi >= 0.0) {
-                ptc = m.ten_off;
-                if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
-            } else {
-                /* negative plastic strain is reset (plastic_BI :143-151); a
-                   negative friction angle otherwise is an error the host
-                   checks before the run */
-                plasticStrain = 0.0;
-                a.ps[(size_t)t * E + e] = 0.0;
-                pphi = m.fa[0];
-                ppsi = m.da[0];
-                pcoh = m.coh[0];
-                coh = a.cohoff ? pcoh + cohoff : pcoh;
+    const bool is_pl = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC);
+    const bool is_vp = (RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC);
+    double ps = 0.0;
+    Soft sp = {0.0, 0.0, 0.0, 0.0};
+    if (RSET & (RS_PLASTIC | RS_VISCOPLASTIC)) {
+        /* own segment match, then resolve the carried parameters in tet order */
+        if ((is_pl || is_vp) && active) ps = a.ps[(size_t)t * E + ec];
+        double matched = 0.0;
+        Soft own = {0.0, 0.0, 0.0, 0.0};
+        if (is_pl || is_vp) {
+            for (uint32_t i = 0; i < m.nseg; i++) {
+                const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
+                if (ps >= pl1 && ps <= pl2) {
+                    const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
+                    const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
+                    const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+                    own.phi = m.fa[i] + tgf * (ps - pl1);
+                    own.psi = m.da[i] + tgd * (ps - pl1);
+                    own.coh = m.coh[i] + tgc * (ps - pl1);
+                    own.hard = tgc;
+                    matched = 1.0;
+                }
             }
-            double sphi, spsi;
-            angle_terms(m, pphi, ppsi, sphi, spsi);
-            double dps = 0.0;
-            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, 0.0, false, dps)) {
-                a.ps[(size_t)t * E + e] = plasticStrain + dps;
-                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
-            }
-        } else if ((RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)) {
-            double plasticStrain = a.ps[(size_t)t * E + e];
-            double trace_strain = sn[0] + sn[1] + sn[2];
-            double d0, d1, d2;
-            devstrain(sn, trace_strain, d0, d1, d2);
-            double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
-            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
-            if (a.visc) a.visc[(size_t)t * E + e] = visc;
-            maxwell_update(m, st, sn, trace_strain, visc, dt);
-            /* unsymmetrised: the lower triangle is zero (viscoplastic_BI :246) */
-            double S[3][3] = {{st[0], st[3], st[4]}, {0.0, st[1], st[5]}, {0.0, 0.0, st[2]}};
-            double d3[3], cn[3][3];
-            principal_stresses(S, d3, cn);
-            soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
-            double coh = a.cohoff ? pcoh + cohoff : pcoh;
-            if (pphi >= 0.0) {
-                ptc = m.ten_off;
-                if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
-            } else {
-                plasticStrain = 0.0;
-                a.ps[(size_t)t * E + e] = 0.0;
-                pphi = m.fa[0];
-                ppsi = m.da[0];
-                pcoh = m.coh[0];
-                coh = a.cohoff ? pcoh + cohoff : pcoh;
-            }
-            double sphi, spsi;
-            angle_terms(m, pphi, ppsi, sphi, spsi);
-            double dps = 0.0;
-            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, phard, true, dps)) {
-                a.ps[(size_t)t * E + e] = plasticStrain + dps;
-                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
-            }
-        } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
-            /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
-            double Xp[4][3];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                int nd = node_index(a, ei + kHexOff[kTetraToNode[t][q]][0], ej + kHexOff[kTetraToNode[t][q]][1],
-                                    ek + kHexOff[kTetraToNode[t][q]][2]);
-                Xp[q][0] = a.xp[nd];
-                Xp[q][1] = a.xp[a.N + nd];
-                Xp[q][2] = a.xp[2 * a.N + nd];
-            }
-            double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
-            double trace_strain = vol[t] / old_volume - 1.0;
-            double d0, d1, d2;
-            devstrain(sn, trace_strain, d0, d1, d2);
-            double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
-            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
-            if (a.visc) a.visc[(size_t)t * E + e] = visc;
-            maxwell_update(m, st, sn, trace_strain, visc, dt);
-        } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
-            const double a2 = m.lambda;
-            const double sh2 = 2.0 * m.mu;
-            double trace_strain = sn[0] + sn[1] + sn[2];
-            st[0] += sh2 * sn[0] + a2 * (trace_strain);
-            st[1] += sh2 * sn[1] + a2 * (trace_strain);
-            st[2] += sh2 * sn[2] + a2 * (trace_strain);
-            st[3] += sh2 * sn[3];
-            st[4] += sh2 * sn[4];
-            st[5] += sh2 * sn[5];
         }
-#pragma unroll
-        for (int c = 0; c < 6; c++) sig[t][c] = st[c];
+        if (in_wave) {
+            sh_sp[le][t][0] = matched;
+            sh_sp[le][t][1] = own.phi;
+            sh_sp[le][t][2] = own.psi;
+            sh_sp[le][t][3] = own.coh;
+            sh_sp[le][t][4] = own.hard;
+            sh_ps[le][t] = ps;
+        }
+        __syncthreads();
+        if (is_pl || is_vp) {
+            sp.phi = is_pl ? 30.0 : 0.0;   /* plastic_BI :78-80 defaults */
+            sp.coh = is_pl ? 30.0e6 : 0.0;
+            for (int q = 0; q <= t; q++) {
+                if (sh_sp[le][q][0] != 0.0) {
+                    sp.phi = sh_sp[le][q][1];
+                    sp.psi = sh_sp[le][q][2];
+                    sp.coh = sh_sp[le][q][3];
+                    sp.hard = sh_sp[le][q][4];
+                }
+                if (!(sp.phi >= 0.0) && sh_ps[le][q] < 0.0) {
+                    /* negative plastic strain is reset (plastic_BI :143-151) */
+                    sp.phi = m.fa[0];
+                    sp.psi = m.da[0];
+                    sp.coh = m.coh[0];
+                    if (q == t) ps = 0.0;
+                }
+            }
+        }
     }
 
-    /* pass 3: stress trace averaging (Stress.c:87-137) */
-    double pst = 0.0, tr[NTET];
+    if (is_pl) {
+        double trace_strain = sn[0] + sn[1] + sn[2];
+        double S[3][3];
+        st[0] += (2.0 * m.mu) * sn[0] + m.lambda * (trace_strain);
+        st[1] += (2.0 * m.mu) * sn[1] + m.lambda * (trace_strain);
+        st[2] += (2.0 * m.mu) * sn[2] + m.lambda * (trace_strain);
+        st[3] += (2.0 * m.mu) * sn[3];
+        st[4] += (2.0 * m.mu) * sn[4];
+        st[5] += (2.0 * m.mu) * sn[5];
+        S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
+        S[0][1] = S[1][0] = st[3];
+        S[0][2] = S[2][0] = st[4];
+        S[1][2] = S[2][1] = st[5];
+        double d3[3], cn[3][3];
+        principal_stresses(S, d3, cn);
+        const double coh = a.cohoff ? sp.coh + a.cohoff[ec] : sp.coh;
+        double tc = m.ten_off;
+        if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
+        double sphi, spsi, dps = 0.0;
+        angle_terms(m, sp.phi, sp.psi, sphi, spsi);
+        if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
+            ps = ps + dps;
+            st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+            st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+        }
+        if (active) a.ps[(size_t)t * E + ec] = ps;
+    } else if (is_vp) {
+        double trace_strain = sn[0] + sn[1] + sn[2];
+        double d0, d1, d2;
+        devstrain(sn, trace_strain, d0, d1, d2);
+        double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
+        double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+        if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
+        maxwell_update(m, st, sn, trace_strain, visc, dt);
+        /* unsymmetrised: the lower triangle is zero (viscoplastic_BI :246) */
+        double S[3][3] = {{st[0], st[3], st[4]}, {0.0, st[1], st[5]}, {0.0, 0.0, st[2]}};
+        double d3[3], cn[3][3];
+        principal_stresses(S, d3, cn);
+        const double coh = a.cohoff ? sp.coh + a.cohoff[ec] : sp.coh;
+        double tc = m.ten_off;
+        if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
+        double sphi, spsi, dps = 0.0;
+        angle_terms(m, sp.phi, sp.psi, sphi, spsi);
+        if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, sp.hard, true, dps)) {
+            ps = ps + dps;
+            st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+            st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+        }
+        if (active) a.ps[(size_t)t * E + ec] = ps;
+    } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
+        /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
+        double Xp[4][3];
 #pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        tr[t] = (sig[t][0] + sig[t][1] + sig[t][2]) / 3.0;
-        pst += tr[t] * vol[t];
+        for (int q = 0; q < 4; q++) {
+            Xp[q][0] = a.xp[nd[q]];
+            Xp[q][1] = a.xp[a.N + nd[q]];
+            Xp[q][2] = a.xp[2 * a.N + nd[q]];
+        }
+        double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
+        double trace_strain = k.V / old_volume - 1.0;
+        double d0, d1, d2;
+        devstrain(sn, trace_strain, d0, d1, d2);
+        double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
+        double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+        if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
+        maxwell_update(m, st, sn, trace_strain, visc, dt);
+    } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
+        const double a2 = m.lambda;
+        const double sh2 = 2.0 * m.mu;
+        double trace_strain = sn[0] + sn[1] + sn[2];
+        st[0] += sh2 * sn[0] + a2 * (trace_strain);
+        st[1] += sh2 * sn[1] + a2 * (trace_strain);
+        st[2] += sh2 * sn[2] + a2 * (trace_strain);
+        st[3] += sh2 * sn[3];
+        st[4] += sh2 * sn[4];
+        st[5] += sh2 * sn[5];
     }
+
+    /* stress trace averaging (Stress.c:87-137) */
+    const double tr = (st[0] + st[1] + st[2]) / 3.0;
+    __syncthreads();
+    if (in_wave) sh_b[le][t] = tr * k.V;
+    __syncthreads();
+    double pst = 0.0;
+#pragma unroll
+    for (int q = 0; q < NTET; q++) pst += sh_b[le][q];
     const double avgTrace = pst / elemVolume;
+    st[0] += -1.0 * tr + avgTrace;
+    st[1] += -1.0 * tr + avgTrace;
+    st[2] += -1.0 * tr + avgTrace;
+    if (active) {
 #pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        sig[t][0] += -1.0 * tr[t] + avgTrace;
-        sig[t][1] += -1.0 * tr[t] + avgTrace;
-        sig[t][2] += -1.0 * tr[t] + avgTrace;
-#pragma unroll
-        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + e] = sig[t][c];
+        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + ec] = st[c];
     }
 }
 
@@ -935,6 +991,43 @@ struct snh_context {
     bool needs_plastic;
     bool needs_visc;
     uint32_t timeStep;
+    /* kernel timing */
+    bool profiling;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int kind; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    size_t ev_used;
+};
+
+enum { KK_ELEM = 0, KK_NODE = 1, KK_GEO = 2, KK_FIN = 3 };
+
+static hipEvent_t take_event(snh_context* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+/* brackets one launch with events when profiling is on */
+struct KTimer {
+    snh_context* c;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(snh_context* c_, int k) : c(c_), kind(k) {
+        if (c->profiling) {
+            a = take_event(c);
+            b = take_event(c);
+            if (a && b) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~KTimer() {
+        if (c->profiling && a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->recs.push_back({kind, a, b});
+        }
+    }
 };
 
 extern "C" int snh_abi_version(void) { return SNH_ABI_VERSION; }
@@ -1200,6 +1293,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
 extern "C" int snh_context_destroy(snh_context* c) {
     if (!c) return SNH_OK;
     (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     free_all(c);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1278,6 +1372,7 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
 }
 
 static int launch_geo(snh_context* c) {
+    KTimer kt(c, KK_GEO);
     StepArgs a = make_args(c);
     hipLaunchKernelGGL(k_geo, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
@@ -1285,6 +1380,7 @@ static int launch_geo(snh_context* c) {
 }
 
 static int launch_fin(snh_context* c) {
+    KTimer kt(c, KK_FIN);
     StepArgs a = make_args(c);
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, a, c->n_geo_blocks, c->n_node_blocks);
     HIPCHK(hipGetLastError());
@@ -1292,8 +1388,9 @@ static int launch_fin(snh_context* c) {
 }
 
 static int launch_elem(snh_context* c) {
+    KTimer kt(c, KK_ELEM);
     StepArgs a = make_args(c);
-    dim3 grid((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), block(ELEM_BLOCK);
+    dim3 grid((c->E + EPW - 1) / EPW), block(64);
     switch (c->rset) {
     case RS_ELASTIC: hipLaunchKernelGGL(k_elem<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
     case RS_PLASTIC: hipLaunchKernelGGL(k_elem<RS_PLASTIC>, grid, block, 0, c->stream, a); break;
@@ -1309,6 +1406,7 @@ static int launch_elem(snh_context* c) {
 }
 
 static int launch_node(snh_context* c) {
+    KTimer kt(c, KK_NODE);
     StepArgs a = make_args(c);
     hipLaunchKernelGGL(k_node, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
@@ -1525,5 +1623,66 @@ extern "C" int snh_download_state(snh_context* c, snh_state* s) {
             }
         }
     }
+    return SNH_OK;
+}
+
+extern "C" int snh_profile(snh_context* c, int enable) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->recs.clear();
+    c->ev_used = 0;
+    c->profiling = enable != 0;
+    return SNH_OK;
+}
+
+extern "C" int snh_get_kernel_times(snh_context* c, snh_kernel_times* out) {
+    if (!c || !out) return set_err(SNH_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double ms[4] = {0, 0, 0, 0};
+    uint32_t n[4] = {0, 0, 0, 0};
+    for (auto& r : c->recs) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.kind] += t;
+        n[r.kind]++;
+    }
+    out->elem_ms = ms[KK_ELEM]; out->node_ms = ms[KK_NODE]; out->geo_ms = ms[KK_GEO]; out->fin_ms = ms[KK_FIN];
+    out->elem_n = n[KK_ELEM]; out->node_n = n[KK_NODE]; out->geo_n = n[KK_GEO]; out->fin_n = n[KK_FIN];
+    return SNH_OK;
+}
+
+__global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int snh_measure_copy_bandwidth(size_t nbytes, int iters, double* gbps) {
+    if (!gbps || nbytes < 1024 || iters < 1) return set_err(SNH_EINVAL, "bad arguments");
+    size_t n = nbytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void**)&a, n * sizeof(double2)));
+    if (hipMalloc((void**)&b, n * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(a);
+        return set_err(SNH_ENOMEM, "copy probe allocation failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemset(a, 0, n * sizeof(double2));
+    int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e0, 0);
+    for (int it = 0; it < iters; it++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return set_err(SNH_EHIP, "copy probe failed: %s", hipGetErrorString(err));
+    *gbps = 2.0 * (double)(n * sizeof(double2)) * iters / (ms * 1e-3) / 1e9;
     return SNH_OK;
 }

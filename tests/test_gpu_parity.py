@@ -45,3 +45,56 @@ def test_maxwell_small():
     p = P.maxwell_viscoelastic(elements=(10, 4, 10))
     ref, got, rsc, gsc = run_both(p, 60)
     _check("maxwell", ref, got, rsc, gsc)
+
+
+def _viscoplastic_problem(elements=(12, 6, 8)):
+    from snac_amd import _abi
+    p = P.mohr_coulomb_extension(elements=elements, seed=2)
+    m = p.materials[0]
+    m.rheology = _abi.RHEOLOGY_VISCOPLASTIC
+    m.refvisc = 1.0e21
+    m.activationE = 0.0
+    m.srexponent = 1.0
+    p.name = "viscoplastic_extension"
+    return p
+
+
+def test_viscoplastic_small():
+    p = _viscoplastic_problem()
+    ref, got, rsc, gsc = run_both(p, 60)
+    _check("vp", ref, got, rsc, gsc)
+
+
+def test_granular_entry_points_match_fused_step():
+    """calc_stresses / loop_nodes / loop_elements / sync one by one give the
+    same state as snh_step."""
+    from snac_amd import Context
+    p = P.mohr_coulomb_extension(elements=(10, 5, 6), seed=4)
+    a = Context(p).Construct().Build().Initialise()
+    a.Execute(15)
+    sa = a.state()
+    a.Destroy()
+    b = Context(p).Construct().Build().Initialise()
+    for _ in range(15):
+        b.calc_stresses()
+        b.loop_nodes_momentum()
+        b.loop_elements_momentum()
+        b.sync()
+    sb = b.state()
+    b.Destroy()
+    for f in ("coord", "velocity", "stress", "plastic_strain"):
+        assert np.array_equal(sa[f], sb[f]), f
+
+
+def test_repeat_runs_bit_identical():
+    """The gather is atomic-free: two runs give identical bits."""
+    from snac_amd import Context
+    p = P.mohr_coulomb_extension(elements=(16, 8, 8), seed=5)
+    outs = []
+    for _ in range(2):
+        c = Context(p).Construct().Build().Initialise()
+        c.Execute(30)
+        outs.append(c.state())
+        c.Destroy()
+    for f in ("coord", "velocity", "stress", "plastic_strain"):
+        assert np.array_equal(outs[0][f], outs[1][f]), f
