@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores given to this job")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -81,6 +82,7 @@ def main():
 
     from snac_amd import Context, _abi, metrics
     problem = make_problem(args.workload)
+    problem.precision = _abi.PRECISION_MIXED if args.precision == "mixed" else _abi.PRECISION_FP64
     E = problem.n_elements
 
     ctx = Context(problem).Construct().Build().Initialise()
@@ -126,10 +128,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": problem.name, "elements": list(problem.elements),
-                   "nodes": list(problem.nodes), "precision": "fp64"},
+                   "nodes": list(problem.nodes), "precision": args.precision},
         "roofline": {
             "bound": "hbm",
             "kernel": {"elem": "k_elem", "node": "k_node", "geo": "k_geo"}[dom],

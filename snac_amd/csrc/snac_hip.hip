@@ -373,7 +373,7 @@ __device__ __forceinline__ bool mc_return(const DevMaterial& m, double st[3][3],
         dep2 = 0.0;
         dep3 = alam;
     }
-    double depm = (dep1 + dep2 + dep3) / 3.0;
+    double depm = snh_div3(dep1 + dep2 + dep3);
     dps = sqrt(0.5 * ((dep1 - depm) * (dep1 - depm) + (dep2 - depm) * (dep2 - depm) +
                       (dep3 - depm) * (dep3 - depm) + depm * depm));
 #pragma unroll
@@ -425,15 +425,16 @@ __device__ __forceinline__ double viscosity_of(const DevMaterial& m, double rvis
 /* deviatoric Maxwell update shared by maxwell and viscoplastic_BI */
 __device__ __forceinline__ void maxwell_update(const DevMaterial& m, double st[6], const double sn[6],
                                                double trace_strain, double visc, double dt) {
-    const double bulkm = m.lambda + 2.0 * m.mu / 3.0;
+    const double bulkm = m.lambda + snh_div3(2.0 * m.mu);
     const double rmu = m.mu;
     double trace_stress = st[0] + st[1] + st[2];
-    double straind0 = sn[0] - (trace_strain) / 3.0;
-    double straind1 = sn[1] - (trace_strain) / 3.0;
-    double straind2 = sn[2] - (trace_strain) / 3.0;
-    double stressd0 = st[0] - (trace_stress) / 3.0;
-    double stressd1 = st[1] - (trace_stress) / 3.0;
-    double stressd2 = st[2] - (trace_stress) / 3.0;
+    const double ts3 = snh_div3(trace_stress), te3 = snh_div3(trace_strain);
+    double straind0 = sn[0] - te3;
+    double straind1 = sn[1] - te3;
+    double straind2 = sn[2] - te3;
+    double stressd0 = st[0] - ts3;
+    double stressd1 = st[1] - ts3;
+    double stressd2 = st[2] - ts3;
     double temp = rmu / (2.0 * visc) * dt;
     double vic1 = 1.0 - temp;
     double vic2 = 1.0 / (1.0 + temp);
@@ -443,7 +444,7 @@ __device__ __forceinline__ void maxwell_update(const DevMaterial& m, double st[6
     st[3] = (st[3] * vic1 + 2.0 * rmu * sn[3]) * vic2;
     st[4] = (st[4] * vic1 + 2.0 * rmu * sn[4]) * vic2;
     st[5] = (st[5] * vic1 + 2.0 * rmu * sn[5]) * vic2;
-    double VolumicStress = trace_stress / 3.0 + bulkm * trace_strain;
+    double VolumicStress = ts3 + bulkm * trace_strain;
     st[0] = stressd0 + VolumicStress;
     st[1] = stressd1 + VolumicStress;
     st[2] = stressd2 + VolumicStress;
@@ -451,9 +452,10 @@ __device__ __forceinline__ void maxwell_update(const DevMaterial& m, double st[6
 
 /* deviatoric strains of a strain tensor for the viscosity law */
 __device__ __forceinline__ void devstrain(const double sn[6], double trace_strain, double& d0, double& d1, double& d2) {
-    d0 = sn[0] - (trace_strain) / 3.0;
-    d1 = sn[1] - (trace_strain) / 3.0;
-    d2 = sn[2] - (trace_strain) / 3.0;
+    const double te3 = snh_div3(trace_strain);
+    d0 = sn[0] - te3;
+    d1 = sn[1] - te3;
+    d2 = sn[2] - te3;
 }
 
 /* --------------------------------------------------------------------------
@@ -536,7 +538,7 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
     }
     TetKin k;
     tet_kinematics(Xt, Ut, k);
-    const double trace = (k.sr[0] + k.sr[1] + k.sr[2]) / 3.0;
+    const double trace = snh_div3(k.sr[0] + k.sr[1] + k.sr[2]);
     if (in_wave) {
         sh_a[le][t] = k.V;
         sh_b[le][t] = trace * k.V;
@@ -696,7 +698,7 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
     }
 
     /* stress trace averaging (Stress.c:87-137) */
-    const double tr = (st[0] + st[1] + st[2]) / 3.0;
+    const double tr = snh_div3(st[0] + st[1] + st[2]);
     __syncthreads();
     if (in_wave) sh_b[le][t] = tr * k.V;
     __syncthreads();
@@ -798,7 +800,8 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
                            X[kTetraToNode[t][kFaceToNode[f][2]]], ax, ay, az);
                 const double sq = sqrt(ax * ax + ay * ay + az * az);
                 const double area = sq / 2.0;
-                const double n0 = ax / sq, n1 = ay / sq, n2 = az / sq;
+                const double ysq = 1.0 / sq;
+                const double n0 = snh_div_by(ax, sq, ysq), n1 = snh_div_by(ay, sq, ysq), n2 = snh_div_by(az, sq, ysq);
                 const double s00 = a.stress[((size_t)0 * NTET + t) * E + e];
                 const double s11 = a.stress[((size_t)1 * NTET + t) * E + e];
                 const double s22 = a.stress[((size_t)2 * NTET + t) * E + e];
@@ -819,9 +822,9 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
                 } else {
                     im += 0.25 * rho * V;
                 }
-                f0 += (s00 * n0 + s01 * n1 + s02 * n2) * area / 6.0;
-                f1 += (s01 * n0 + s11 * n1 + s12 * n2) * area / 6.0;
-                f2 += (s02 * n0 + s12 * n1 + s22 * n2) * area / 6.0;
+                f0 += snh_div6((s00 * n0 + s01 * n1 + s02 * n2) * area);   /* (...)*area/6.0 */
+                f1 += snh_div6((s01 * n0 + s11 * n1 + s12 * n2) * area);
+                f2 += snh_div6((s02 * n0 + s12 * n1 + s22 * n2) * area);
                 f1 -= a.gravity * rho * V / 8.0;
             }
         }
@@ -866,6 +869,8 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_vmax(StepArgs a) {
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
 }
+
+#include "snh_mixed.h"
 
 /* --------------------------------------------------------------------------
  * k_geo: Snac_EP_LoopElementsMomentum's geometry and minLengthScale
@@ -1121,8 +1126,16 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0: surface diffusion is order dependent and not on the path");
     if (cfg->dtType == SNH_DT_WAVE) return set_err(SNH_EUNSUPPORTED, "dtType wave is not implemented on the device");
     if (cfg->dtType != SNH_DT_DYNAMIC && cfg->dtType != SNH_DT_CONSTANT) return set_err(SNH_EINVAL, "bad dtType");
-    if (cfg->precision != SNH_PRECISION_FP64)
-        return set_err(SNH_EUNSUPPORTED, "only SNH_PRECISION_FP64 is built in this library version");
+    if (cfg->precision != SNH_PRECISION_FP64 && cfg->precision != SNH_PRECISION_MIXED)
+        return set_err(SNH_EINVAL, "bad precision");
+    if (cfg->precision == SNH_PRECISION_MIXED) {
+        if (cfg->dtType != SNH_DT_DYNAMIC)
+            return set_err(SNH_EUNSUPPORTED, "mixed precision runs the dynamic dt type only");
+        for (unsigned q = 0; q < cfg->nmaterials; q++)
+            if ((cfg->materials[q].rheology & SNH_RHEOLOGY_VISCOPLASTIC) &&
+                !(cfg->materials[q].rheology & SNH_RHEOLOGY_PLASTIC))
+                return set_err(SNH_EUNSUPPORTED, "mixed precision has no visco-plastic law yet");
+    }
     if (cfg->nbcs && !cfg->bcs) return set_err(SNH_EINVAL, "nbcs > 0 with null bcs");
     double vol = (double)cfg->meshSizeI * cfg->meshSizeJ * cfg->meshSizeK;
     if (vol > 2.0e9) return set_err(SNH_EINVAL, "mesh too large for 32-bit node indices");
@@ -1391,6 +1404,18 @@ static int launch_elem(snh_context* c) {
     KTimer kt(c, KK_ELEM);
     StepArgs a = make_args(c);
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
+    if (c->cfg.precision == SNH_PRECISION_MIXED) {
+        switch (c->rset) {
+        case RS_ELASTIC: hipLaunchKernelGGL(k_elem_mx<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
+        case RS_PLASTIC: hipLaunchKernelGGL(k_elem_mx<RS_PLASTIC>, grid, block, 0, c->stream, a); break;
+        case RS_MAXWELL: hipLaunchKernelGGL(k_elem_mx<RS_MAXWELL>, grid, block, 0, c->stream, a); break;
+        default:
+            hipLaunchKernelGGL(k_elem_mx<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL>, grid, block, 0, c->stream, a);
+            break;
+        }
+        HIPCHK(hipGetLastError());
+        return SNH_OK;
+    }
     switch (c->rset) {
     case RS_ELASTIC: hipLaunchKernelGGL(k_elem<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
     case RS_PLASTIC: hipLaunchKernelGGL(k_elem<RS_PLASTIC>, grid, block, 0, c->stream, a); break;
@@ -1408,7 +1433,10 @@ static int launch_elem(snh_context* c) {
 static int launch_node(snh_context* c) {
     KTimer kt(c, KK_NODE);
     StepArgs a = make_args(c);
-    hipLaunchKernelGGL(k_node, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    if (c->cfg.precision == SNH_PRECISION_MIXED)
+        hipLaunchKernelGGL(k_node_mx, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_node, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     c->cur ^= 1;
     return SNH_OK;

@@ -115,7 +115,7 @@ SNH_HD void face_cross(const double* c1, const double* c2, const double* c3, dou
    face 0 (nodes 0,1,2): x3*y2 == y2*x3 exactly. */
 SNH_HD double tet_volume_from_face0(const double* c1, const double* c4, double a0, double b0, double c0) {
     double x4 = c4[0] - c1[0], y4 = c4[1] - c1[1], z4 = c4[2] - c1[2];
-    double v = (x4 * a0 + y4 * b0 + z4 * c0) / 6.0;
+    double v = snh_div6(x4 * a0 + y4 * b0 + z4 * c0);   /* == (...)/6.0 */
     return fabs(v);
 }
 
@@ -140,10 +140,11 @@ SNH_HD void tet_kinematics(const double X[4][3], const double U[4][3], TetKin& o
             c0 = c;
         }
         double s = sqrt(a * a + b * b + c * c);
+        const double ys = 1.0 / s;
         A[f] = s / 2.0;
-        n[f][0] = a / s;
-        n[f][1] = b / s;
-        n[f][2] = c / s;
+        n[f][0] = snh_div_by(a, s, ys);   /* == a / s */
+        n[f][1] = snh_div_by(b, s, ys);
+        n[f][2] = snh_div_by(c, s, ys);
     }
     const double V = tet_volume_from_face0(X[0], X[3], a0, b0, c0);
     o.V = V;
@@ -155,16 +156,19 @@ SNH_HD void tet_kinematics(const double X[4][3], const double U[4][3], TetKin& o
     const double s01 = T_(0, 0, 1) + T_(1, 0, 1) + T_(2, 0, 1) + T_(3, 0, 1);
     const double s02 = T_(0, 0, 2) + T_(1, 0, 2) + T_(2, 0, 2) + T_(3, 0, 2);
     const double s12 = T_(0, 1, 2) + T_(1, 1, 2) + T_(2, 1, 2) + T_(3, 1, 2);
-    o.sr[0] = -1.0 * s00 / 3.0 / V;
-    o.sr[1] = -1.0 * s11 / 3.0 / V;
-    o.sr[2] = -1.0 * s22 / 3.0 / V;
-    o.sr[3] = -0.5 * (s01 + T_(0, 1, 0) + T_(1, 1, 0) + T_(2, 1, 0) + T_(3, 1, 0)) / 3.0 / V;
-    o.sr[4] = -0.5 * (s02 + T_(0, 2, 0) + T_(1, 2, 0) + T_(2, 2, 0) + T_(3, 2, 0)) / 3.0 / V;
-    o.sr[5] = -0.5 * (s12 + T_(0, 2, 1) + T_(1, 2, 1) + T_(2, 2, 1) + T_(3, 2, 1)) / 3.0 / V;
-    const double hv = -(0.5 / V);
-    o.w[0] = hv * (s01 - T_(0, 1, 0) - T_(1, 1, 0) - T_(2, 1, 0) - T_(3, 1, 0)) / 3.0;
-    o.w[1] = hv * (s02 - T_(0, 2, 0) - T_(1, 2, 0) - T_(2, 2, 0) - T_(3, 2, 0)) / 3.0;
-    o.w[2] = hv * (s12 - T_(0, 2, 1) - T_(1, 2, 1) - T_(2, 2, 1) - T_(3, 2, 1)) / 3.0;
+    /* x / 3.0 / V as snh_div_by(snh_div3(x), V, RN(1/V)): the same two
+       correctly rounded quotients */
+    const double yV = 1.0 / V;
+    o.sr[0] = snh_div_by(snh_div3(-1.0 * s00), V, yV);
+    o.sr[1] = snh_div_by(snh_div3(-1.0 * s11), V, yV);
+    o.sr[2] = snh_div_by(snh_div3(-1.0 * s22), V, yV);
+    o.sr[3] = snh_div_by(snh_div3(-0.5 * (s01 + T_(0, 1, 0) + T_(1, 1, 0) + T_(2, 1, 0) + T_(3, 1, 0))), V, yV);
+    o.sr[4] = snh_div_by(snh_div3(-0.5 * (s02 + T_(0, 2, 0) + T_(1, 2, 0) + T_(2, 2, 0) + T_(3, 2, 0))), V, yV);
+    o.sr[5] = snh_div_by(snh_div3(-0.5 * (s12 + T_(0, 2, 1) + T_(1, 2, 1) + T_(2, 2, 1) + T_(3, 2, 1))), V, yV);
+    const double hv = -(yV * 0.5);   /* RN(0.5/V) == RN(1/V)/2 */
+    o.w[0] = snh_div3(hv * (s01 - T_(0, 1, 0) - T_(1, 1, 0) - T_(2, 1, 0) - T_(3, 1, 0)));
+    o.w[1] = snh_div3(hv * (s02 - T_(0, 2, 0) - T_(1, 2, 0) - T_(2, 2, 0) - T_(3, 2, 0)));
+    o.w[2] = snh_div3(hv * (s12 - T_(0, 2, 1) - T_(1, 2, 1) - T_(2, 2, 1) - T_(3, 2, 1)));
 #undef T_
 }
 

@@ -64,4 +64,37 @@ SNH_HD double snh_hypot(double x, double y) {
     return h * scale;
 }
 
+/* RN(x / 3) without a division.  x = M 2^e with integer M, and ulp(x/3) <=
+ * ulp(x), so x/3 measured in ulps of the result has fractional part 0, 1/3 or
+ * 2/3: the exact quotient is at least 1/6 ulp from every rounding midpoint.
+ * q = RN(x * RN(1/3)) is within 1.5 ulp; r = x - 3q is then a multiple of
+ * ulp(q) of magnitude <= 3 ulp(q), so the FMA computes it exactly; and
+ * q + r*RN(1/3) differs from x/3 by r*(RN(1/3) - 1/3) < 2^-53 ulp, which
+ * cannot move it across a midpoint.  Hence one rounding gives RN(x/3), for all
+ * normal x (zero keeps its magnitude; -0 may come out as +0). */
+SNH_HD double snh_div3(double x) {
+    const double y = 0x1.5555555555555p-2; /* RN(1/3) */
+    double q = x * y;
+    double r = fma(-3.0, q, x);
+    return fma(r, y, q);
+}
+
+/* RN(x / 6) = RN(x / 3) / 2 exactly (halving commutes with rounding in the
+ * normal range) */
+SNH_HD double snh_div6(double x) { return snh_div3(x) * 0.5; }
+
+/* RN(a / b) from y = RN(1 / b) (P. Markstein, "Computation of elementary
+ * functions on the IBM RISC System/6000", IBM J. Res. Dev. 34 (1990); and
+ * Muller et al., Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.7):
+ * q0 = RN(a y) is within 1.5 ulp; one residual step makes it faithful; with
+ * a faithful q and the correctly rounded reciprocal, r = a - b q is exact and
+ * RN(q + r y) is the correctly rounded quotient.  Normal range only. */
+SNH_HD double snh_div_by(double a, double b, double y) {
+    double q = a * y;
+    double r = fma(-b, q, a);
+    q = fma(r, y, q);
+    r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
 #endif

@@ -1,0 +1,411 @@
+/*
+ * snh_mixed.h -- SNH_PRECISION_MIXED kernels: strain-rate, rotation,
+ * constitutive-increment and force arithmetic in FP32; node coordinates,
+ * velocities, tet stresses and plastic strain stay FP64.
+ *
+ * Per-step increments are ~1e-8 of the state they update, so every FP32
+ * quantity is an increment, a geometry term built from FP64 coordinate
+ * differences, or a difference of stresses taken in FP64 first:
+ *   - element geometry: edge vectors relative to the tet's first node are
+ *     formed in FP64 and rounded once to FP32;
+ *   - stress rotation and the elastic / Maxwell updates are FP32 increments
+ *     added to the FP64 stress;
+ *   - trace averaging and the mean stress stay FP64;
+ *   - Maxwell's volumetric strain V/V_old - 1 (a difference of nearly equal
+ *     volumes) is formed in FP64;
+ *   - the Mohr-Coulomb eigen-solve and return run in FP64 on the FP64 trial
+ *     stress;
+ *   - the nodal force is gathered relative to a reference stress per
+ *     surrounding element (its first tet): the reference part
+ *     sigma_ref . sum(n A)/6 is exact FP64, the remainder, small stress
+ *     differences times FP32 face vectors, is summed in FP64.
+ * Within one build the result is deterministic (atomic-free, fixed order).
+ */
+#ifndef SNH_MIXED_H
+#define SNH_MIXED_H
+
+struct TetKinF {
+    float V;
+    float sr[6];
+    float w[3];
+};
+
+/* E: tet node coordinates relative to its node 0 (E[0] = 0), U: velocities */
+__device__ __forceinline__ void tet_kinematics_f(const float E[4][3], const float U[4][3], TetKinF& o) {
+#pragma clang fp contract(fast)
+    float n[4][3], A[4];
+    float a0 = 0.f, b0 = 0.f, c0 = 0.f;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const float* p0 = E[kFaceToNode[f][0]];
+        const float* p1 = E[kFaceToNode[f][1]];
+        const float* p2 = E[kFaceToNode[f][2]];
+        float x2 = p1[0] - p0[0], y2 = p1[1] - p0[1], z2 = p1[2] - p0[2];
+        float x3 = p2[0] - p0[0], y3 = p2[1] - p0[1], z3 = p2[2] - p0[2];
+        float a = y2 * z3 - z2 * y3, b = z2 * x3 - x2 * z3, c = x2 * y3 - y2 * x3;
+        if (f == 0) {
+            a0 = a;
+            b0 = b;
+            c0 = c;
+        }
+        float s = __builtin_sqrtf(a * a + b * b + c * c);
+        float is = 1.0f / s;
+        A[f] = 0.5f * s;
+        n[f][0] = a * is;
+        n[f][1] = b * is;
+        n[f][2] = c * is;
+    }
+    const float V = fabsf((E[3][0] * a0 + E[3][1] * b0 + E[3][2] * c0) * (1.0f / 6.0f));
+    o.V = V;
+    const float iV3 = 1.0f / (3.0f * V);
+#define TF(k, p, q) (U[k][p] * n[((k) + 1) & 3][q] * A[((k) + 1) & 3])
+    const float s00 = TF(0, 0, 0) + TF(1, 0, 0) + TF(2, 0, 0) + TF(3, 0, 0);
+    const float s11 = TF(0, 1, 1) + TF(1, 1, 1) + TF(2, 1, 1) + TF(3, 1, 1);
+    const float s22 = TF(0, 2, 2) + TF(1, 2, 2) + TF(2, 2, 2) + TF(3, 2, 2);
+    const float s01 = TF(0, 0, 1) + TF(1, 0, 1) + TF(2, 0, 1) + TF(3, 0, 1);
+    const float s02 = TF(0, 0, 2) + TF(1, 0, 2) + TF(2, 0, 2) + TF(3, 0, 2);
+    const float s12 = TF(0, 1, 2) + TF(1, 1, 2) + TF(2, 1, 2) + TF(3, 1, 2);
+    const float s10 = TF(0, 1, 0) + TF(1, 1, 0) + TF(2, 1, 0) + TF(3, 1, 0);
+    const float s20 = TF(0, 2, 0) + TF(1, 2, 0) + TF(2, 2, 0) + TF(3, 2, 0);
+    const float s21 = TF(0, 2, 1) + TF(1, 2, 1) + TF(2, 2, 1) + TF(3, 2, 1);
+#undef TF
+    o.sr[0] = -s00 * iV3;
+    o.sr[1] = -s11 * iV3;
+    o.sr[2] = -s22 * iV3;
+    o.sr[3] = -0.5f * (s01 + s10) * iV3;
+    o.sr[4] = -0.5f * (s02 + s20) * iV3;
+    o.sr[5] = -0.5f * (s12 + s21) * iV3;
+    o.w[0] = -0.5f * (s01 - s10) * iV3;
+    o.w[1] = -0.5f * (s02 - s20) * iV3;
+    o.w[2] = -0.5f * (s12 - s21) * iV3;
+}
+
+/* Jaumann rotation as FP32 increments applied in the reference's order to
+   the FP64 stress (UpdateElement.c:245-265) */
+__device__ __forceinline__ void rotate_stress_mx(double st[6], const float w[3], float dt) {
+#pragma clang fp contract(fast)
+    const float w01 = w[0] * dt, w02 = w[1] * dt, w12 = w[2] * dt;
+    st[0] += (double)(2.0f * (w01 * (float)st[3] + w02 * (float)st[4]));
+    st[1] += (double)(2.0f * (-w01 * (float)st[3] + w12 * (float)st[5]));
+    st[2] += (double)(2.0f * (-w02 * (float)st[4] - w12 * (float)st[5]));
+    st[3] += (double)(w01 * (float)(st[1] - st[0]) + w02 * (float)st[5] + w12 * (float)st[4]);
+    st[4] += (double)(w02 * (float)(st[2] - st[0]) + w01 * (float)st[5] - w12 * (float)st[3]);
+    st[5] += (double)(w12 * (float)(st[2] - st[1]) - w01 * (float)st[4] - w02 * (float)st[3]);
+}
+
+template <int RSET>
+__global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
+    __shared__ double sh_a[EPW][NTET];
+    __shared__ double sh_b[EPW][NTET];
+    __shared__ double sh_sp[EPW][NTET][5];
+    __shared__ double sh_ps[EPW][NTET];
+    const int lane = threadIdx.x;
+    const bool in_wave = lane < EPW * NTET;
+    const int le = in_wave ? lane / NTET : 0;
+    const int t = in_wave ? lane - le * NTET : 0;
+    const int e = blockIdx.x * EPW + le;
+    const bool active = in_wave && e < a.E;
+    const size_t E = (size_t)a.E;
+    const int ec = active ? e : 0;
+    const int ei = ec % a.EI, ej = (ec / a.EI) % a.EJ, ek = ec / (a.EI * a.EJ);
+    const double dt = a.scal->dt;
+    const float dtf = (float)dt;
+    const float dt_rotf = (float)a.scal->dt_rot;
+    const DevMaterial& m = c_mat[a.mat ? a.mat[ec] : 0];
+
+    int nd[4];
+    double X0[3];
+    float Ef[4][3], Uf[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        nd[q] = hex_node_index(a, ei, ej, ek, tet_node(t, q));
+        const double x = a.xc[nd[q]], y = a.xc[a.N + nd[q]], z = a.xc[2 * a.N + nd[q]];
+        if (q == 0) {
+            X0[0] = x;
+            X0[1] = y;
+            X0[2] = z;
+        }
+        Ef[q][0] = (float)(x - X0[0]);
+        Ef[q][1] = (float)(y - X0[1]);
+        Ef[q][2] = (float)(z - X0[2]);
+        Uf[q][0] = (float)a.v[nd[q]];
+        Uf[q][1] = (float)a.v[a.N + nd[q]];
+        Uf[q][2] = (float)a.v[2 * a.N + nd[q]];
+    }
+    TetKinF k;
+    tet_kinematics_f(Ef, Uf, k);
+    const float trace = (k.sr[0] + k.sr[1] + k.sr[2]) * (1.0f / 3.0f);
+    if (in_wave) {
+        sh_a[le][t] = (double)k.V;
+        sh_b[le][t] = (double)(trace * k.V);
+    }
+    __syncthreads();
+    double evol = 0.0, partial = 0.0;
+#pragma unroll
+    for (int q = 0; q < NTET; q++) {
+        evol += sh_a[le][q];
+        partial += sh_b[le][q];
+    }
+    const double elemVolume = evol;
+    const float srcorr = (float)(partial / elemVolume);
+
+    double st[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) st[c] = active ? a.stress[((size_t)c * NTET + t) * E + ec] : 0.0;
+    rotate_stress_mx(st, k.w, dt_rotf);
+    float sn[6];
+#pragma unroll
+    for (int c = 0; c < 3; c++) sn[c] = (k.sr[c] - trace + srcorr) * dtf;
+#pragma unroll
+    for (int c = 3; c < 6; c++) sn[c] = k.sr[c] * dtf;
+
+    const bool is_pl = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC);
+    double ps = 0.0;
+    Soft sp = {0.0, 0.0, 0.0, 0.0};
+    if (RSET & RS_PLASTIC) {
+        if (is_pl && active) ps = a.ps[(size_t)t * E + ec];
+        double matched = 0.0;
+        Soft own = {0.0, 0.0, 0.0, 0.0};
+        if (is_pl) {
+            for (uint32_t i = 0; i < m.nseg; i++) {
+                const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
+                if (ps >= pl1 && ps <= pl2) {
+                    const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
+                    const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
+                    const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+                    own.phi = m.fa[i] + tgf * (ps - pl1);
+                    own.psi = m.da[i] + tgd * (ps - pl1);
+                    own.coh = m.coh[i] + tgc * (ps - pl1);
+                    own.hard = tgc;
+                    matched = 1.0;
+                }
+            }
+        }
+        if (in_wave) {
+            sh_sp[le][t][0] = matched;
+            sh_sp[le][t][1] = own.phi;
+            sh_sp[le][t][2] = own.psi;
+            sh_sp[le][t][3] = own.coh;
+            sh_sp[le][t][4] = own.hard;
+            sh_ps[le][t] = ps;
+        }
+        __syncthreads();
+        if (is_pl) {
+            sp.phi = 30.0;
+            sp.coh = 30.0e6;
+            for (int q = 0; q <= t; q++) {
+                if (sh_sp[le][q][0] != 0.0) {
+                    sp.phi = sh_sp[le][q][1];
+                    sp.psi = sh_sp[le][q][2];
+                    sp.coh = sh_sp[le][q][3];
+                    sp.hard = sh_sp[le][q][4];
+                }
+                if (!(sp.phi >= 0.0) && sh_ps[le][q] < 0.0) {
+                    sp.phi = m.fa[0];
+                    sp.psi = m.da[0];
+                    sp.coh = m.coh[0];
+                    if (q == t) ps = 0.0;
+                }
+            }
+        }
+    }
+
+    if (is_pl) {
+        const float trs = sn[0] + sn[1] + sn[2];
+        const float twomu = (float)(2.0 * m.mu), lam = (float)m.lambda;
+        st[0] += (double)(twomu * sn[0] + lam * trs);
+        st[1] += (double)(twomu * sn[1] + lam * trs);
+        st[2] += (double)(twomu * sn[2] + lam * trs);
+        st[3] += (double)(twomu * sn[3]);
+        st[4] += (double)(twomu * sn[4]);
+        st[5] += (double)(twomu * sn[5]);
+        double S[3][3];
+        S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
+        S[0][1] = S[1][0] = st[3];
+        S[0][2] = S[2][0] = st[4];
+        S[1][2] = S[2][1] = st[5];
+        double d3[3], cn[3][3];
+        principal_stresses(S, d3, cn);
+        const double coh = a.cohoff ? sp.coh + a.cohoff[ec] : sp.coh;
+        double tc = m.ten_off;
+        if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
+        double sphi, spsi, dps = 0.0;
+        angle_terms(m, sp.phi, sp.psi, sphi, spsi);
+        if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
+            ps = ps + dps;
+            st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+            st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+        }
+        if (active) a.ps[(size_t)t * E + ec] = ps;
+    } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
+        /* volumetric strain from FP64 volumes of the two configurations */
+        double Xc[4][3], Xp[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            Xc[q][0] = a.xc[nd[q]];
+            Xc[q][1] = a.xc[a.N + nd[q]];
+            Xc[q][2] = a.xc[2 * a.N + nd[q]];
+            Xp[q][0] = a.xp[nd[q]];
+            Xp[q][1] = a.xp[a.N + nd[q]];
+            Xp[q][2] = a.xp[2 * a.N + nd[q]];
+        }
+        const double v_new = tet_volume(Xc[0], Xc[1], Xc[2], Xc[3]);
+        const double v_old = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
+        const double trace_strain = (v_new - v_old) / v_old;
+        const double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
+        double visc;
+        {
+            const double te3 = trace_strain / 3.0;
+            double d0 = sn[0] - te3, d1 = sn[1] - te3, d2 = sn[2] - te3;
+            const double snd[6] = {sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]};
+            visc = viscosity_of(m, rvisc, d0, d1, d2, snd, dt);
+        }
+        if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
+        /* sigma'_new = sigma' + sigma'(-2x/(1+x)) + 2 mu e'/(1+x), x = mu dt/(2 eta) */
+        const double mean = (st[0] + st[1] + st[2]) / 3.0;
+        const float x = (float)(m.mu / (2.0 * visc) * dt);
+        const float fdec = -2.0f * x / (1.0f + x);
+        const float gmu = (float)(2.0 * m.mu) / (1.0f + x);
+        const float te3f = (float)(trace_strain / 3.0);
+        const double bulk_inc = (m.lambda + 2.0 * m.mu / 3.0) * trace_strain;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float sd = (float)(st[c] - mean);
+            st[c] += (double)(sd * fdec + gmu * (sn[c] - te3f)) + bulk_inc;
+        }
+#pragma unroll
+        for (int c = 3; c < 6; c++) st[c] += (double)((float)st[c] * fdec + gmu * sn[c]);
+    } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
+        const float trs = sn[0] + sn[1] + sn[2];
+        const float twomu = (float)(2.0 * m.mu), lam = (float)m.lambda;
+        st[0] += (double)(twomu * sn[0] + lam * trs);
+        st[1] += (double)(twomu * sn[1] + lam * trs);
+        st[2] += (double)(twomu * sn[2] + lam * trs);
+        st[3] += (double)(twomu * sn[3]);
+        st[4] += (double)(twomu * sn[4]);
+        st[5] += (double)(twomu * sn[5]);
+    }
+
+    /* stress trace averaging in FP64 (Stress.c:87-137) */
+    const double tr = (st[0] + st[1] + st[2]) / 3.0;
+    __syncthreads();
+    if (in_wave) sh_b[le][t] = tr * (double)k.V;
+    __syncthreads();
+    double pst = 0.0;
+#pragma unroll
+    for (int q = 0; q < NTET; q++) pst += sh_b[le][q];
+    const double avgTrace = pst / elemVolume;
+    st[0] += -tr + avgTrace;
+    st[1] += -tr + avgTrace;
+    st[2] += -tr + avgTrace;
+    if (active) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + ec] = st[c];
+    }
+}
+
+/* node gather, mixed precision */
+__global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
+    __shared__ double sh[NODE_BLOCK / 64];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double vmag = -INFINITY;
+    if (n < a.N) {
+        const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+        const double dt = a.scal->dt;
+        const double c = a.scal->speedOfSound;
+        const size_t E = (size_t)a.E;
+        const size_t NN = (size_t)a.N;
+        const double xn0 = a.xc[n], xn1 = a.xc[NN + n], xn2 = a.xc[2 * NN + n];
+        double im = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+#pragma unroll
+        for (int slot = 0; slot < 8; slot++) {
+            const int ei = i - 1 + (slot & 1), ej = j - 1 + ((slot >> 1) & 1), ek = k - 1 + ((slot >> 2) & 1);
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= a.EI || ej >= a.EJ || ek >= a.EK) continue;
+            const int e = ei + a.EI * (ej + a.EJ * ek);
+            const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+            const float idens = (float)(m.lam2mu / (c * c));
+            const float rho = (float)m.rho_eff;
+            const float grav8 = (float)a.gravity * rho * 0.125f;
+            double X[8][3];
+            load_hex(a, a.xc, ei, ej, ek, X);
+            float Ef[8][3];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                Ef[q][0] = (float)(X[q][0] - xn0);
+                Ef[q][1] = (float)(X[q][1] - xn1);
+                Ef[q][2] = (float)(X[q][2] - xn2);
+            }
+            /* reference stress of this element: its first listed tet */
+            const int tref = kNodeToTetra[slot][0];
+            double sref[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) sref[q] = a.stress[((size_t)q * NTET + tref) * E + e];
+            double C0 = 0.0, C1 = 0.0, C2 = 0.0;   /* sum of FP64 face vectors (2 n A) */
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f, mi = 0.f, gr = 0.f;
+#pragma unroll
+            for (int idx = 0; idx < 5; idx++) {
+#pragma clang fp contract(fast)
+                const int t = kNodeToTetra[slot][idx];
+                const int f = kNodeToSurface[slot][idx];
+                const int q0 = kTetraToNode[t][kFaceToNode[f][0]];
+                const int q1 = kTetraToNode[t][kFaceToNode[f][1]];
+                const int q2 = kTetraToNode[t][kFaceToNode[f][2]];
+                double dax, day, daz;
+                face_cross(X[q0], X[q1], X[q2], dax, day, daz);
+                C0 += dax;
+                C1 += day;
+                C2 += daz;
+                /* FP32 geometry of the tet */
+                const float* p0 = Ef[kTetraToNode[t][0]];
+                const float* p1 = Ef[kTetraToNode[t][1]];
+                const float* p2 = Ef[kTetraToNode[t][2]];
+                const float* p3 = Ef[kTetraToNode[t][3]];
+                float x2 = p1[0] - p0[0], y2 = p1[1] - p0[1], z2 = p1[2] - p0[2];
+                float x3 = p2[0] - p0[0], y3 = p2[1] - p0[1], z3 = p2[2] - p0[2];
+                float x4 = p3[0] - p0[0], y4 = p3[1] - p0[1], z4 = p3[2] - p0[2];
+                const float V = fabsf((x4 * (y2 * z3 - z2 * y3) + y4 * (z2 * x3 - x2 * z3) + z4 * (x2 * y3 - x3 * y2)) *
+                                      (1.0f / 6.0f));
+                const float ax = (float)dax, ay = (float)day, az = (float)daz;   /* 2 n A */
+                float ds[6];
+#pragma unroll
+                for (int q = 0; q < 6; q++) ds[q] = (float)(a.stress[((size_t)q * NTET + t) * E + e] - sref[q]);
+                mi += 0.25f * idens * V;
+                g0 += ds[0] * ax + ds[3] * ay + ds[4] * az;
+                g1 += ds[3] * ax + ds[1] * ay + ds[5] * az;
+                g2 += ds[4] * ax + ds[5] * ay + ds[2] * az;
+                gr += grav8 * V;
+            }
+            /* sigma . n A / 6 = sigma . (2 n A) / 12 */
+            f0 += (sref[0] * C0 + sref[3] * C1 + sref[4] * C2) / 12.0 + (double)g0 / 12.0;
+            f1 += (sref[3] * C0 + sref[1] * C1 + sref[5] * C2) / 12.0 + (double)g1 / 12.0 - (double)gr;
+            f2 += (sref[4] * C0 + sref[5] * C1 + sref[2] * C2) / 12.0 + (double)g2 / 12.0;
+            if (a.dtType == SNH_DT_DYNAMIC) im += (double)mi;
+        }
+        if (a.dtType != SNH_DT_DYNAMIC) im = 1.0; /* host rejects other dt types in mixed mode */
+        double v0 = a.v[n], v1 = a.v[NN + n], v2 = a.v[2 * NN + n];
+        f0 -= a.demf * GETSIGN(v0) * fabs(f0);
+        f1 -= a.demf * GETSIGN(v1) * fabs(f1);
+        f2 -= a.demf * GETSIGN(v2) * fabs(f2);
+        v0 += dt * f0 / im;
+        v1 += dt * f1 / im;
+        v2 += dt * f2 / im;
+        const uint32_t code = a.bc[n];
+        if (code) {
+            const DevBC& b = c_bc[code];
+            if (b.mask & 1u) v0 = b.val[0];
+            if (b.mask & 2u) v1 = b.val[1];
+            if (b.mask & 4u) v2 = b.val[2];
+            vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        }
+        a.v[n] = v0;
+        a.v[NN + n] = v1;
+        a.v[2 * NN + n] = v2;
+        a.xn[n] = xn0 + dt * v0;
+        a.xn[NN + n] = xn1 + dt * v1;
+        a.xn[2 * NN + n] = xn2 + dt * v2;
+    }
+    double bm = block_max(vmag, sh);
+    if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+}
+
+#endif
