@@ -35,15 +35,14 @@ def cpu_model():
     return platform.processor()
 
 
-def make_problem(name):
+def make_problem(name, elements=""):
     from snac_amd import problem as P
-    if name == "mohr_coulomb_extension":
-        return P.mohr_coulomb_extension()
-    if name == "elastic_box_compression":
-        return P.elastic_box_compression()
-    if name == "maxwell_viscoelastic":
-        return P.maxwell_viscoelastic()
-    raise SystemExit(f"unknown workload {name}")
+    kw = {}
+    if elements:
+        kw["elements"] = tuple(int(v) for v in elements.lower().split("x"))
+    if name not in P.WORKLOADS:
+        raise SystemExit(f"unknown workload {name}")
+    return P.WORKLOADS[name](**kw)
 
 
 def cpu_baseline(problem, steps, threads):
@@ -73,6 +72,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores given to this job")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--elements", default="", help="override the mesh, e.g. 256x64x128")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -81,11 +81,12 @@ def main():
         sys.exit(2)
 
     from snac_amd import Context, _abi, metrics
-    problem = make_problem(args.workload)
+    problem = make_problem(args.workload, args.elements)
     problem.precision = _abi.PRECISION_MIXED if args.precision == "mixed" else _abi.PRECISION_FP64
     E = problem.n_elements
 
     ctx = Context(problem).Construct().Build().Initialise()
+    tune = ctx.autotune()
     ctx.step(args.warmup)
     ctx.synchronize()
 
@@ -152,6 +153,7 @@ def main():
             "kernel_sum_ms_per_step": kernel_sum_ms,
         },
         "final_dt": sc["dt"],
+        "autotune": tune,
     }
 
     if not args.no_cpu and args.cpu_steps > 0:

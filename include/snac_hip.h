@@ -230,6 +230,22 @@ typedef struct snh_kernel_times {
 } snh_kernel_times;
 int snh_profile(snh_context* ctx, int enable);
 
+/* Kernel-variant autotuner (no counterpart in the reference).  Times each
+ * candidate implementation of the element pass (one tet per lane, or one
+ * element per thread) on a scratch copy of the current state, a few launches
+ * each, and keeps the fastest for this context.  Call after
+ * snh_timestep_zero; the state is not modified.  The choice can be forced
+ * with the environment variable SNH_ELEM_VARIANT=0|1. */
+#define SNH_ELEM_TET_PER_LANE     0
+#define SNH_ELEM_ELEMENT_PER_LANE 1
+typedef struct snh_tune_report {
+    double   elem_ms[2];     /* average launch time per variant, ms (0 = not run) */
+    int32_t  elem_variant;   /* chosen */
+} snh_tune_report;
+int snh_autotune(snh_context* ctx, snh_tune_report* out);
+/* Force one element-pass variant (SNH_ELEM_*); both give identical results. */
+int snh_set_elem_variant(snh_context* ctx, int32_t variant);
+
 /* Device-to-device copy bandwidth of the current GPU (GB/s, read + write
  * bytes), from a grid-stride copy of nbytes timed with HIP events over iters
  * launches: the measured HBM peak reported beside the 8 TB/s specification.

@@ -120,6 +120,14 @@ class KernelTimes(C.Structure):
                 ("elem_n", C.c_uint32), ("node_n", C.c_uint32), ("geo_n", C.c_uint32), ("fin_n", C.c_uint32)]
 
 
+ELEM_TET_PER_LANE = 0
+ELEM_ELEMENT_PER_LANE = 1
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("elem_ms", C.c_double * 2), ("elem_variant", C.c_int32)]
+
+
 # Every function the header declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -143,6 +151,8 @@ SIGNATURES = {
     "snh_get_scalars": (_I, [_P, C.POINTER(Scalars)]),
     "snh_synchronize": (_I, [_P]),
     "snh_profile": (_I, [_P, _I]),
+    "snh_autotune": (_I, [_P, C.POINTER(TuneReport)]),
+    "snh_set_elem_variant": (_I, [_P, C.c_int32]),
     "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }

@@ -105,6 +105,18 @@ class Context:
     def synchronize(self):
         self._check(self.lib.snh_synchronize(self.h))
 
+    def autotune(self) -> Dict[str, float]:
+        """Pick the fastest element-pass variant for this mesh and law."""
+        r = _abi.TuneReport()
+        self._check(self.lib.snh_autotune(self.h, C.byref(r)))
+        return {"elem_ms_tet_per_lane": r.elem_ms[0], "elem_ms_element_per_lane": r.elem_ms[1],
+                "elem_variant": r.elem_variant}
+
+    def set_elem_variant(self, variant: int):
+        """Force the element-pass variant (tet per lane or element per lane)."""
+        self._check(self.lib.snh_set_elem_variant(self.h, int(variant)))
+        return self
+
     def profile(self, enable: bool = True):
         self._check(self.lib.snh_profile(self.h, int(enable)))
 

@@ -22,6 +22,7 @@
 
 #include <math.h>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -716,6 +717,208 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
 }
 
 /* --------------------------------------------------------------------------
+ * k_elem_seq: the same element pass with one thread per element, the 10 tets
+ * in sequence (kept as a second variant: for laws dominated by per-tet
+ * iterative work, e.g. the plastic eigen-solve, one element per lane
+ * diverges less than one tet per lane).  Identical arithmetic.
+ * ------------------------------------------------------------------------ */
+template <int RSET>
+__global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    const double dt = a.scal->dt;
+    const double dt_rot = a.scal->dt_rot;
+    const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+
+    double X[8][3], U[8][3];
+    load_hex(a, a.xc, ei, ej, ek, X);
+    load_hex(a, a.v, ei, ej, ek, U);
+
+    /* pass 1: kinematics of the 10 tets */
+    double sr[NTET][6], w[NTET][3], vol[NTET];
+    double evol = 0.0;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        double Xt[4][3], Ut[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                Xt[q][d] = X[kTetraToNode[t][q]][d];
+                Ut[q][d] = U[kTetraToNode[t][q]][d];
+            }
+        TetKin k;
+        tet_kinematics(Xt, Ut, k);
+        vol[t] = k.V;
+        evol += k.V;
+#pragma unroll
+        for (int c = 0; c < 6; c++) sr[t][c] = k.sr[c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) w[t][c] = k.w[c];
+    }
+    evol *= 0.5;                        /* UpdateElement.c:267-268 */
+    const double elemVolume = 2.0 * evol; /* StrainRate.c:57, Stress.c:68 */
+    double partial = 0.0;
+    double trace[NTET];
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        trace[t] = snh_div3(sr[t][0] + sr[t][1] + sr[t][2]);
+        partial += trace[t] * vol[t];
+    }
+    const double srcorr = partial / elemVolume;
+
+    /* pass 2: rotation, strain, constitutive */
+    double sig[NTET][6];
+    const size_t E = (size_t)a.E;
+    /* plastic_BI / viscoplastic_BI keep their softened parameters across the
+       tets of an element */
+    double pcoh = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0e6 : 0.0;
+    double pphi = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0 : 0.0;
+    double ppsi = 0.0, phard = 0.0, ptc = 0.0;
+    const double cohoff = a.cohoff ? a.cohoff[e] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        double st[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) st[c] = a.stress[((size_t)c * NTET + t) * E + e];
+        rotate_stress(st, w[t], dt_rot);
+        double s6[6];
+#pragma unroll
+        for (int c = 0; c < 3; c++) s6[c] = sr[t][c] + (-1.0 * trace[t] + srcorr);
+#pragma unroll
+        for (int c = 3; c < 6; c++) s6[c] = sr[t][c];
+        double sn[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) sn[c] = s6[c] * dt;
+
+        if ((RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC)) {
+            double plasticStrain = a.ps[(size_t)t * E + e];
+            double trace_strain = sn[0] + sn[1] + sn[2];
+            double S[3][3];
+            st[0] += (2.0 * m.mu) * sn[0] + m.lambda * (trace_strain);
+            st[1] += (2.0 * m.mu) * sn[1] + m.lambda * (trace_strain);
+            st[2] += (2.0 * m.mu) * sn[2] + m.lambda * (trace_strain);
+            st[3] += (2.0 * m.mu) * sn[3];
+            st[4] += (2.0 * m.mu) * sn[4];
+            st[5] += (2.0 * m.mu) * sn[5];
+            S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
+            S[0][1] = S[1][0] = st[3];
+            S[0][2] = S[2][0] = st[4];
+            S[1][2] = S[2][1] = st[5];
+            double d3[3], cn[3][3];
+            principal_stresses(S, d3, cn);
+            soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
+            double coh = a.cohoff ? pcoh + cohoff : pcoh;
+            if (pphi >= 0.0) {
+                ptc = m.ten_off;
+                if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
+            } else {
+                /* negative plastic strain is reset (plastic_BI :143-151); a
+                   negative friction angle otherwise is an error the host
+                   checks before the run */
+                plasticStrain = 0.0;
+                a.ps[(size_t)t * E + e] = 0.0;
+                pphi = m.fa[0];
+                ppsi = m.da[0];
+                pcoh = m.coh[0];
+                coh = a.cohoff ? pcoh + cohoff : pcoh;
+            }
+            double sphi, spsi;
+            angle_terms(m, pphi, ppsi, sphi, spsi);
+            double dps = 0.0;
+            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, 0.0, false, dps)) {
+                a.ps[(size_t)t * E + e] = plasticStrain + dps;
+                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            }
+        } else if ((RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)) {
+            double plasticStrain = a.ps[(size_t)t * E + e];
+            double trace_strain = sn[0] + sn[1] + sn[2];
+            double d0, d1, d2;
+            devstrain(sn, trace_strain, d0, d1, d2);
+            double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
+            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+            if (a.visc) a.visc[(size_t)t * E + e] = visc;
+            maxwell_update(m, st, sn, trace_strain, visc, dt);
+            /* unsymmetrised: the lower triangle is zero (viscoplastic_BI :246) */
+            double S[3][3] = {{st[0], st[3], st[4]}, {0.0, st[1], st[5]}, {0.0, 0.0, st[2]}};
+            double d3[3], cn[3][3];
+            principal_stresses(S, d3, cn);
+            soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
+            double coh = a.cohoff ? pcoh + cohoff : pcoh;
+            if (pphi >= 0.0) {
+                ptc = m.ten_off;
+                if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
+            } else {
+                plasticStrain = 0.0;
+                a.ps[(size_t)t * E + e] = 0.0;
+                pphi = m.fa[0];
+                ppsi = m.da[0];
+                pcoh = m.coh[0];
+                coh = a.cohoff ? pcoh + cohoff : pcoh;
+            }
+            double sphi, spsi;
+            angle_terms(m, pphi, ppsi, sphi, spsi);
+            double dps = 0.0;
+            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, phard, true, dps)) {
+                a.ps[(size_t)t * E + e] = plasticStrain + dps;
+                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            }
+        } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
+            /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
+            double Xp[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int nd = node_index(a, ei + kHexOff[kTetraToNode[t][q]][0], ej + kHexOff[kTetraToNode[t][q]][1],
+                                    ek + kHexOff[kTetraToNode[t][q]][2]);
+                Xp[q][0] = a.xp[nd];
+                Xp[q][1] = a.xp[a.N + nd];
+                Xp[q][2] = a.xp[2 * a.N + nd];
+            }
+            double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
+            double trace_strain = vol[t] / old_volume - 1.0;
+            double d0, d1, d2;
+            devstrain(sn, trace_strain, d0, d1, d2);
+            double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
+            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+            if (a.visc) a.visc[(size_t)t * E + e] = visc;
+            maxwell_update(m, st, sn, trace_strain, visc, dt);
+        } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
+            const double a2 = m.lambda;
+            const double sh2 = 2.0 * m.mu;
+            double trace_strain = sn[0] + sn[1] + sn[2];
+            st[0] += sh2 * sn[0] + a2 * (trace_strain);
+            st[1] += sh2 * sn[1] + a2 * (trace_strain);
+            st[2] += sh2 * sn[2] + a2 * (trace_strain);
+            st[3] += sh2 * sn[3];
+            st[4] += sh2 * sn[4];
+            st[5] += sh2 * sn[5];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) sig[t][c] = st[c];
+    }
+
+    /* pass 3: stress trace averaging (Stress.c:87-137) */
+    double pst = 0.0, tr[NTET];
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        tr[t] = snh_div3(sig[t][0] + sig[t][1] + sig[t][2]);
+        pst += tr[t] * vol[t];
+    }
+    const double avgTrace = pst / elemVolume;
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        sig[t][0] += -1.0 * tr[t] + avgTrace;
+        sig[t][1] += -1.0 * tr[t] + avgTrace;
+        sig[t][2] += -1.0 * tr[t] + avgTrace;
+#pragma unroll
+        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + e] = sig[t][c];
+    }
+}
+
+/* --------------------------------------------------------------------------
  * k_rotated: stress with the pending rotation applied, for downloads
  * ------------------------------------------------------------------------ */
 __global__ void __launch_bounds__(ELEM_BLOCK) k_rotated(StepArgs a, double* __restrict__ out) {
@@ -996,6 +1199,7 @@ struct snh_context {
     bool needs_plastic;
     bool needs_visc;
     uint32_t timeStep;
+    int elem_variant;   /* SNH_ELEM_* */
     /* kernel timing */
     bool profiling;
     std::vector<hipEvent_t> ev_pool;
@@ -1296,6 +1500,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         return rc;
     }
     c->cur = 0;
+    c->elem_variant = SNH_ELEM_TET_PER_LANE;
+    if (const char* ev = getenv("SNH_ELEM_VARIANT")) c->elem_variant = atoi(ev) == 1 ? 1 : 0;
     c->zero_done = false;
     c->rotation_pending = false;
     c->timeStep = 0;
@@ -1374,6 +1580,13 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
         HIPCHK(hipMemcpy(c->stress, buf.data(), 60 * E * sizeof(double), hipMemcpyHostToDevice));
         c->rotation_pending = false;
     }
+    if ((s->coord || s->velocity || s->stress) && c->zero_done) {
+        /* uploaded arrays are the state after LoopElementsMomentum (stress
+           already rotated): cancel the pending rotation */
+        const double zero = 0.0;
+        HIPCHK(hipMemcpy((char*)c->scal + offsetof(DevScalars, dt_rot), &zero, sizeof zero, hipMemcpyHostToDevice));
+        c->rotation_pending = false;
+    }
     if (s->plastic_strain) {
         if (!c->ps) return set_err(SNH_EINVAL, "no plastic rheology in this context");
         buf.resize(10 * E);
@@ -1400,9 +1613,12 @@ static int launch_fin(snh_context* c) {
     return SNH_OK;
 }
 
+static int launch_elem_args(snh_context* c, StepArgs a);
 static int launch_elem(snh_context* c) {
     KTimer kt(c, KK_ELEM);
-    StepArgs a = make_args(c);
+    return launch_elem_args(c, make_args(c));
+}
+static int launch_elem_args(snh_context* c, StepArgs a) {
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
     if (c->cfg.precision == SNH_PRECISION_MIXED) {
         switch (c->rset) {
@@ -1411,6 +1627,21 @@ static int launch_elem(snh_context* c) {
         case RS_MAXWELL: hipLaunchKernelGGL(k_elem_mx<RS_MAXWELL>, grid, block, 0, c->stream, a); break;
         default:
             hipLaunchKernelGGL(k_elem_mx<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL>, grid, block, 0, c->stream, a);
+            break;
+        }
+        HIPCHK(hipGetLastError());
+        return SNH_OK;
+    }
+    if (c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE) {
+        dim3 g2((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), b2(ELEM_BLOCK);
+        switch (c->rset) {
+        case RS_ELASTIC: hipLaunchKernelGGL(k_elem_seq<RS_ELASTIC>, g2, b2, 0, c->stream, a); break;
+        case RS_PLASTIC: hipLaunchKernelGGL(k_elem_seq<RS_PLASTIC>, g2, b2, 0, c->stream, a); break;
+        case RS_MAXWELL: hipLaunchKernelGGL(k_elem_seq<RS_MAXWELL>, g2, b2, 0, c->stream, a); break;
+        case RS_VISCOPLASTIC: hipLaunchKernelGGL(k_elem_seq<RS_VISCOPLASTIC>, g2, b2, 0, c->stream, a); break;
+        default:
+            hipLaunchKernelGGL(k_elem_seq<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL | RS_VISCOPLASTIC>, g2, b2, 0,
+                               c->stream, a);
             break;
         }
         HIPCHK(hipGetLastError());
@@ -1712,5 +1943,75 @@ extern "C" int snh_measure_copy_bandwidth(size_t nbytes, int iters, double* gbps
     (void)hipFree(b);
     if (err != hipSuccess) return set_err(SNH_EHIP, "copy probe failed: %s", hipGetErrorString(err));
     *gbps = 2.0 * (double)(n * sizeof(double2)) * iters / (ms * 1e-3) / 1e9;
+    return SNH_OK;
+}
+
+extern "C" int snh_set_elem_variant(snh_context* c, int32_t v) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (v != SNH_ELEM_TET_PER_LANE && v != SNH_ELEM_ELEMENT_PER_LANE) return set_err(SNH_EINVAL, "unknown element variant");
+    c->elem_variant = v;
+    return SNH_OK;
+}
+
+extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_autotune before snh_timestep_zero");
+    snh_tune_report rep;
+    memset(&rep, 0, sizeof rep);
+    const size_t E = c->E;
+    double *st = nullptr, *ps = nullptr, *visc = nullptr;
+    HIPCHK(hipMalloc((void**)&st, 60 * E * sizeof(double)));
+    if (c->ps && hipMalloc((void**)&ps, 10 * E * sizeof(double)) != hipSuccess) {
+        (void)hipFree(st);
+        return set_err(SNH_ENOMEM, "autotune scratch allocation failed");
+    }
+    if (c->visc && hipMalloc((void**)&visc, 10 * E * sizeof(double)) != hipSuccess) {
+        (void)hipFree(st);
+        (void)hipFree(ps);
+        return set_err(SNH_ENOMEM, "autotune scratch allocation failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int saved = c->elem_variant;
+    const int reps = 3;
+    int rc = SNH_OK;
+    for (int v = 0; v < 2 && rc == SNH_OK; v++) {
+        c->elem_variant = v;
+        StepArgs a = make_args(c);
+        a.stress = st;
+        a.ps = ps;
+        a.visc = visc;
+        float best = 1e30f;
+        for (int r = 0; r < reps + 1 && rc == SNH_OK; r++) {
+            /* fresh copy of the state each launch: the kernel updates in place */
+            if (hipMemcpyAsync(st, c->stress, 60 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+                (ps && hipMemcpyAsync(ps, c->ps, 10 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)) {
+                rc = set_err(SNH_EHIP, "autotune copy failed");
+                break;
+            }
+            (void)hipEventRecord(e0, c->stream);
+            rc = launch_elem_args(c, a);
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = set_err(SNH_EHIP, "autotune launch failed");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0 && ms < best) best = ms;   /* first launch is a warm-up */
+        }
+        rep.elem_ms[v] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(st);
+    if (ps) (void)hipFree(ps);
+    if (visc) (void)hipFree(visc);
+    if (rc) {
+        c->elem_variant = saved;
+        return rc;
+    }
+    c->elem_variant = rep.elem_ms[1] < rep.elem_ms[0] ? SNH_ELEM_ELEMENT_PER_LANE : SNH_ELEM_TET_PER_LANE;
+    if (getenv("SNH_ELEM_VARIANT")) c->elem_variant = saved;
+    rep.elem_variant = c->elem_variant;
+    if (out) *out = rep;
     return SNH_OK;
 }

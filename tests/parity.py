@@ -7,7 +7,7 @@ from snac_amd import Context
 FIELDS = ("coord", "velocity", "stress", "plastic_strain")
 
 
-def run_both(problem, nsteps, threads=4, fields=FIELDS):
+def run_both(problem, nsteps, threads=4, fields=FIELDS, variant=None):
     o = Oracle(problem, threads=threads)
     if problem.hydrostatic:
         o.hydrostatic_ic()
@@ -17,6 +17,8 @@ def run_both(problem, nsteps, threads=4, fields=FIELDS):
     rsc = o.scalars()
     o.close()
     ctx = Context(problem).Construct().Build().Initialise()
+    if variant is not None:
+        ctx.set_elem_variant(variant)
     ctx.Execute(nsteps)
     got = ctx.state(fields=fields)
     gsc = ctx.scalars()

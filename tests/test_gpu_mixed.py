@@ -1,6 +1,12 @@
-"""Mixed precision (FP32 arithmetic, FP64 state) against the FP64 path of the
-same inputs: the error is stated per field; the bar is 1e-6 relative on the
-surface topography."""
+"""Mixed precision (FP32 arithmetic, FP64 state) against the FP64 path on the
+same inputs; the error is stated per field.
+
+The reference's local damping, f -= demf sign(v) |f| (UpdateNode.c:102-104),
+is discontinuous in v: with demf = 0.8 a 1e-12 relative perturbation of the
+FP64 initial stress already moves the surface by ~1e-3 of its displacement
+after a few hundred steps.  The arithmetic error of the mixed path is
+therefore measured undamped (demf = 0, bar: 1e-6 relative topography), and
+with damping it is compared with that FP64 self-sensitivity."""
 import numpy as np
 import pytest
 
@@ -12,43 +18,65 @@ pytestmark = pytest.mark.gpu
 TOPO_TOL = 1e-6
 
 
-def field_errors(p, nsteps):
-    out = {}
-    runs = {}
-    for prec in (_abi.PRECISION_FP64, _abi.PRECISION_MIXED):
-        p.precision = prec
-        c = Context(p).Construct().Build().Initialise()
-        x0 = c.state(fields=("coord",))["coord"]
-        c.Execute(nsteps)
-        runs[prec] = (x0, c.state(fields=("coord", "velocity", "stress", "plastic_strain")), c.scalars())
-        c.Destroy()
-    x0, a, sa = runs[_abi.PRECISION_FP64]
-    _, b, sb = runs[_abi.PRECISION_MIXED]
+def _run(p, prec, nsteps, perturb=0.0):
+    p.precision = prec
+    c = Context(p).Construct().Build().Initialise()
+    x0 = c.state(fields=("coord",))["coord"]
+    if perturb:
+        st = c.state(fields=("stress",))["stress"]
+        c.set_state(stress=st * (1.0 + perturb))
+    c.Execute(nsteps)
+    out = c.state(fields=("coord", "velocity", "stress", "plastic_strain"))
+    sc = c.scalars()
+    c.Destroy()
+    return x0, out, sc
+
+
+def _errors(p, x0, a, sa, b, sb):
     I, J, K = p.nodes
     top = (np.arange(p.n_nodes) // I) % J == J - 1
     dy_a = a["coord"][top, 1] - x0[top, 1]
     dy_b = b["coord"][top, 1] - x0[top, 1]
-    out["topography"] = float(np.max(np.abs(dy_a - dy_b)) / np.max(np.abs(dy_a)))
-    disp_a = a["coord"] - x0
-    out["displacement"] = float(np.max(np.abs(b["coord"] - a["coord"])) / np.max(np.abs(disp_a)))
-    out["stress"] = float(np.max(np.abs(a["stress"] - b["stress"])) / np.max(np.abs(a["stress"])))
-    out["velocity"] = float(np.max(np.abs(a["velocity"] - b["velocity"])) / np.max(np.abs(a["velocity"])))
-    out["dt"] = abs(sa["dt"] - sb["dt"]) / sa["dt"]
-    print(f"  {p.name} mixed vs fp64 after {nsteps} steps: " +
-          ", ".join(f"{k}={v:.2e}" for k, v in out.items()), flush=True)
-    return out
+    return {
+        "topography": float(np.max(np.abs(dy_a - dy_b)) / np.max(np.abs(dy_a))),
+        "displacement": float(np.max(np.abs(b["coord"] - a["coord"])) / np.max(np.abs(a["coord"] - x0))),
+        "stress": float(np.max(np.abs(a["stress"] - b["stress"])) / np.max(np.abs(a["stress"]))),
+        "velocity": float(np.max(np.abs(a["velocity"] - b["velocity"])) / np.max(np.abs(a["velocity"]))),
+        "dt": abs(sa["dt"] - sb["dt"]) / sa["dt"],
+    }
 
 
-def test_mixed_maxwell_topography_error():
-    err = field_errors(P.maxwell_viscoelastic(elements=(16, 6, 16)), 300)
+def _report(tag, err):
+    print(f"  {tag}: " + ", ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+
+
+@pytest.mark.parametrize("make", [
+    lambda: P.maxwell_viscoelastic(elements=(16, 6, 16)),
+    lambda: P.elastic_box_compression(elements=(16, 8, 16)),
+], ids=["maxwell", "elastic"])
+def test_mixed_undamped_topography_error(make):
+    p = make()
+    p.demf = 0.0
+    x0, a, sa = _run(p, _abi.PRECISION_FP64, 300)
+    _, b, sb = _run(p, _abi.PRECISION_MIXED, 300)
+    err = _errors(p, x0, a, sa, b, sb)
+    _report(f"{p.name} demf=0 mixed vs fp64, 300 steps", err)
     assert err["topography"] <= TOPO_TOL
 
 
-def test_mixed_elastic_topography_error():
-    err = field_errors(P.elastic_box_compression(elements=(16, 8, 16)), 300)
-    assert err["topography"] <= TOPO_TOL
-
-
-def test_mixed_plastic_runs_and_reports_error():
-    err = field_errors(P.mohr_coulomb_extension(elements=(16, 6, 8), seed=1), 200)
-    assert np.isfinite(err["topography"])
+@pytest.mark.parametrize("make", [
+    lambda: P.maxwell_viscoelastic(elements=(16, 6, 16)),
+    lambda: P.elastic_box_compression(elements=(16, 8, 16)),
+    lambda: P.mohr_coulomb_extension(elements=(16, 6, 8), seed=1),
+], ids=["maxwell", "elastic", "mohr_coulomb"])
+def test_mixed_damped_error_within_fp64_sensitivity(make):
+    p = make()
+    x0, a, sa = _run(p, _abi.PRECISION_FP64, 300)
+    _, b, sb = _run(p, _abi.PRECISION_MIXED, 300)
+    _, c, sc = _run(p, _abi.PRECISION_FP64, 300, perturb=1e-12)
+    mixed = _errors(p, x0, a, sa, b, sb)
+    floor = _errors(p, x0, a, sa, c, sc)
+    _report(f"{p.name} demf=0.8 mixed vs fp64", mixed)
+    _report(f"{p.name} demf=0.8 fp64 with 1e-12 stress perturbation vs fp64", floor)
+    assert np.isfinite(mixed["topography"])
+    assert mixed["topography"] <= max(10.0 * floor["topography"], TOPO_TOL)

@@ -9,6 +9,9 @@ TOL = 1e-12   # FP64 parity bar: relative to the field's max magnitude
 
 pytestmark = pytest.mark.gpu
 
+# both element-pass variants: one tet per lane, one element per lane
+VARIANTS = pytest.mark.parametrize("variant", [0, 1], ids=["tet_lane", "elem_lane"])
+
 
 def _report(name, ref, got, fields=FIELDS):
     out = {}
@@ -28,23 +31,26 @@ def _check(name, ref, got, rsc, gsc, fields=FIELDS):
     assert rsc["minLengthScale"] == pytest.approx(gsc["minLengthScale"], rel=TOL)
 
 
-def test_elastic_box_compression_small():
+@VARIANTS
+def test_elastic_box_compression_small(variant):
     p = P.elastic_box_compression(elements=(8, 4, 8))
-    ref, got, rsc, gsc = run_both(p, 60)
-    _check("elastic", ref, got, rsc, gsc)
+    ref, got, rsc, gsc = run_both(p, 60, variant=variant)
+    _check(f"elastic[{variant}]", ref, got, rsc, gsc)
 
 
-def test_mohr_coulomb_extension_small():
+@VARIANTS
+def test_mohr_coulomb_extension_small(variant):
     p = P.mohr_coulomb_extension(elements=(12, 6, 8), seed=1)
-    ref, got, rsc, gsc = run_both(p, 60)
-    _check("mc", ref, got, rsc, gsc)
+    ref, got, rsc, gsc = run_both(p, 60, variant=variant)
+    _check(f"mc[{variant}]", ref, got, rsc, gsc)
     assert np.max(ref["plastic_strain"]) >= 0.5
 
 
-def test_maxwell_small():
+@VARIANTS
+def test_maxwell_small(variant):
     p = P.maxwell_viscoelastic(elements=(10, 4, 10))
-    ref, got, rsc, gsc = run_both(p, 60)
-    _check("maxwell", ref, got, rsc, gsc)
+    ref, got, rsc, gsc = run_both(p, 60, variant=variant)
+    _check(f"maxwell[{variant}]", ref, got, rsc, gsc)
 
 
 def _viscoplastic_problem(elements=(12, 6, 8)):
@@ -59,10 +65,11 @@ def _viscoplastic_problem(elements=(12, 6, 8)):
     return p
 
 
-def test_viscoplastic_small():
+@VARIANTS
+def test_viscoplastic_small(variant):
     p = _viscoplastic_problem()
-    ref, got, rsc, gsc = run_both(p, 60)
-    _check("vp", ref, got, rsc, gsc)
+    ref, got, rsc, gsc = run_both(p, 60, variant=variant)
+    _check(f"vp[{variant}]", ref, got, rsc, gsc)
 
 
 def test_granular_entry_points_match_fused_step():
@@ -98,3 +105,23 @@ def test_repeat_runs_bit_identical():
         c.Destroy()
     for f in ("coord", "velocity", "stress", "plastic_strain"):
         assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
+def test_autotune_keeps_state_and_results():
+    """The autotuner times both variants on scratch copies: the state and the
+    subsequent steps are the same as without it."""
+    from snac_amd import Context
+    p = P.mohr_coulomb_extension(elements=(12, 6, 8), seed=6)
+    a = Context(p).Construct().Build().Initialise()
+    rep = a.autotune()
+    print(f"  autotune: {rep}", flush=True)
+    assert rep["elem_ms_tet_per_lane"] > 0 and rep["elem_ms_element_per_lane"] > 0
+    a.Execute(20)
+    sa = a.state()
+    a.Destroy()
+    b = Context(p).Construct().Build().Initialise()
+    b.Execute(20)
+    sb = b.state()
+    b.Destroy()
+    for f in ("coord", "velocity", "stress", "plastic_strain"):
+        assert np.array_equal(sa[f], sb[f]), f
