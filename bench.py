@@ -45,21 +45,69 @@ def make_problem(name, elements=""):
     return P.WORKLOADS[name](**kw)
 
 
-def cpu_baseline(problem, steps, threads):
+def cpu_baseline(problem, seconds, threads, max_steps=200):
     """The CPU oracle (C restatement of the reference step, gcc -O2
     -ffp-contract=off, OpenMP over the element and node loops) timed on this
-    host: TimeStepZero, one warm-up step, then `steps` timed steps."""
+    host: TimeStepZero, one warm-up step, then whole steps until `seconds` of
+    wall time have passed (at least two).  Returns (rate, seconds, steps)."""
     from oracle.oracle import Oracle
     o = Oracle(problem, threads=threads)
     if problem.hydrostatic:
         o.hydrostatic_ic()
     o.timestep_zero()
     o.step(1)
+    steps = 0
     t0 = time.perf_counter()
-    o.step(steps)
+    while steps < max_steps:
+        o.step(1)
+        steps += 1
+        if steps >= 2 and time.perf_counter() - t0 >= seconds:
+            break
     t1 = time.perf_counter()
     o.close()
-    return problem.n_elements * steps / (t1 - t0), t1 - t0
+    return problem.n_elements * steps / (t1 - t0), t1 - t0, steps
+
+
+def adjoint_bench(make, param, steps, interval):
+    """Forward time of `steps` plain steps vs the adjoint's reverse sweep
+    (segment replays + reverse kernels) on the same inputs."""
+    import numpy as np
+    from snac_amd import Context
+    from snac_amd.autograd import top_nodes
+    p = make()
+    c = Context(p).Construct().Build().Initialise()
+    c.step(2)
+    c.synchronize()
+    c.Destroy()
+    c = Context(p).Construct().Build().Initialise()
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.synchronize()
+    plain_ms = 1e3 * (time.perf_counter() - t0)
+    c.Destroy()
+    c = Context(p).Construct().Build().Initialise()
+    top = top_nodes(p)
+    x0 = c.state(fields=("coord",))["coord"]
+    c.adjoint_begin(param, interval)
+    c.adjoint_forward(steps)
+    x = c.state(fields=("coord",))["coord"]
+    cb = np.zeros((p.n_nodes, 3))
+    cb[top, 1] = x[top, 1] - x0[top, 1]
+    t0 = time.perf_counter()
+    g = c.adjoint_reverse(cb)
+    sweep_ms = 1e3 * (time.perf_counter() - t0)
+    t = c.adjoint_times()
+    c.adjoint_end()
+    c.Destroy()
+    return {
+        "workload": p.name, "elements": list(p.elements), "steps": steps, "checkpoint_interval": interval,
+        "forward_ms": plain_ms, "recorded_forward_ms": t["forward_ms"], "replay_ms": t["replay_ms"],
+        "reverse_ms": t["reverse_ms"], "sweep_wall_ms": sweep_ms,
+        "ratio": (t["replay_ms"] + t["reverse_ms"]) / plain_ms,
+        "ratio_reverse_kernels": t["reverse_ms"] / plain_ms,
+        "grad_norm": float(np.linalg.norm(g)),
+    }
 
 
 def main():
@@ -68,11 +116,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="mohr_coulomb_extension")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="length of the CPU oracle sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores given to this job")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--elements", default="", help="override the mesh, e.g. 256x64x128")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,12 +206,21 @@ def main():
         "autotune": tune,
     }
 
-    if not args.no_cpu and args.cpu_steps > 0:
+    if not args.no_adjoint and args.precision == "fp64":
+        from snac_amd import problem as P
+        out["adjoint"] = [
+            adjoint_bench(lambda: P.maxwell_viscoelastic(elements=(64, 32, 64)), _abi.PARAM_LOG_VISCOSITY,
+                          args.adjoint_steps, 50),
+            adjoint_bench(lambda: P.mohr_coulomb_shear(elements=(64, 32, 64)), _abi.PARAM_COHESION,
+                          args.adjoint_steps, 50),
+        ]
+
+    if not args.no_cpu and args.cpu_seconds > 0:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        cval, csec = cpu_baseline(problem, args.cpu_steps, threads)
+        cval, csec, csteps = cpu_baseline(problem, args.cpu_seconds, threads)
         out["cpu_baseline"] = {
             "value": cval, "unit": "element-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{problem.name} {problem.elements} full mesh, {args.cpu_steps} timed steps after "
+            "sample": f"{problem.name} {problem.elements} full mesh, {csteps} timed steps after "
                       f"TimeStepZero + 1 warm-up step ({csec:.1f} s), CPU oracle gcc -O2 -ffp-contract=off "
                       f"OpenMP on {cpu_model()}",
         }

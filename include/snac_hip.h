@@ -246,6 +246,64 @@ int snh_autotune(snh_context* ctx, snh_tune_report* out);
 /* Force one element-pass variant (SNH_ELEM_*); both give identical results. */
 int snh_set_elem_variant(snh_context* ctx, int32_t variant);
 
+/* Prescribe, for the next nsteps calls of the step (snh_step; FP64 path),
+ * dt and speed of sound of each step and optionally the damping branch
+ * sign(v) sign(f) of every node component (damp: [nsteps][N][3] in
+ * {-1, 0, 1}, or NULL): sign(v) in the damping term is replaced by
+ * code * sign(f).  A schedule recorded from a run reproduces that run bit
+ * for bit; with perturbed inputs it holds the run's discrete decisions (the
+ * map the adjoint below differentiates).  nsteps = 0 clears it.  No
+ * counterpart in the reference. */
+int snh_set_schedule(snh_context* ctx, int32_t nsteps, const double* dt, const double* speed_of_sound,
+                     const int8_t* damp);
+
+/* ---- adjoint of the explicit step ----------------------------------------
+ * No counterpart in the reference (SNAC has no adjoint).  Reverse-mode
+ * derivative of n explicit steps with respect to a per-element parameter
+ * field, for misfits of the final coordinates and velocities.  The
+ * derivative is that of the step with its discrete decisions frozen at the
+ * recorded trajectory: dt, rotation dt and speed of sound of every step, the
+ * damping branch sign(v) sign(f) of every node component, the plastic branch
+ * of every tet (elastic / shear / tension) and the viscosity clamp.
+ *
+ * Memory: checkpoints of the full state every checkpoint_interval steps, and
+ * a tape of checkpoint_interval steps (stress, plastic strain, coordinates,
+ * velocity, undamped force, mass) that each segment is replayed into before
+ * its reverse sweep: two-level checkpointing, every forward step is run
+ * twice in total.
+ *
+ * Supported: FP64, dynamic dt, elastic, plastic (plastic_BI semantics,
+ * friction and dilation angles constant along the softening table) and
+ * Maxwell with a linear viscosity law (srexponent 1, activationE 0). */
+#define SNH_PARAM_LOG_VISCOSITY 0   /* theta_e = ln(refvisc_e); needs SNH_FIELD_REFVISC */
+#define SNH_PARAM_COHESION      1   /* theta_e = cohesion offset of element e (Pa)      */
+
+typedef struct snh_adjoint_config {
+    int32_t param;                /* SNH_PARAM_*                                  */
+    int32_t checkpoint_interval;  /* steps per checkpoint segment, >= 1           */
+} snh_adjoint_config;
+
+typedef struct snh_adjoint_times {
+    double forward_ms;   /* snh_adjoint_forward, checkpoint copies included */
+    double replay_ms;    /* forward replays of the reverse sweep            */
+    double reverse_ms;   /* reverse kernels                                 */
+    int32_t steps;       /* steps recorded                                  */
+    int32_t checkpoints;
+} snh_adjoint_times;
+
+/* Start recording at the current state (after snh_timestep_zero). */
+int snh_adjoint_begin(snh_context* ctx, const snh_adjoint_config* cfg);
+/* Run nsteps explicit steps (same results as snh_step), storing checkpoints. */
+int snh_adjoint_forward(snh_context* ctx, int32_t nsteps);
+/* Reverse sweep over all recorded steps.  coord_bar / vel_bar: d(misfit)/d
+ * (final coordinates / velocities), [N][3] node-major like snh_state, either
+ * may be NULL (zero).  grad: d(misfit)/d(theta), [E] doubles, written.  The
+ * context's state is left at the end of the forward run. */
+int snh_adjoint_reverse(snh_context* ctx, const double* coord_bar, const double* vel_bar, double* grad);
+int snh_adjoint_get_times(snh_context* ctx, snh_adjoint_times* out);
+/* Release checkpoints, tape and adjoint buffers. */
+int snh_adjoint_end(snh_context* ctx);
+
 /* Device-to-device copy bandwidth of the current GPU (GB/s, read + write
  * bytes), from a grid-stride copy of nbytes timed with HIP events over iters
  * launches: the measured HBM peak reported beside the 8 TB/s specification.

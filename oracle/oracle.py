@@ -89,6 +89,8 @@ def lib():
             "orc_principal_stresses": (None, [DP, DP, DP]),
             "orc_ref_iabs": (I, [D]),
             "orc_set_glibc_hypot": (None, [I]),
+            "orc_record_decisions": (None, [P, C.POINTER(C.c_int8), C.POINTER(C.c_int8)]),
+            "orc_get_bc": (I, [P, C.POINTER(C.c_uint8), DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -147,6 +149,22 @@ class Oracle:
 
     def set_bc_scan(self, faithful: bool):
         lib().orc_set_bc_scan(self.h, int(faithful))
+
+    def record_decisions(self, damp, branch):
+        """Record sign(v) sign(f) per node component ([N, 3] int8) and the
+        plastic branch per tet ([E, 10] int8) during the following steps;
+        None stops recording.  The arrays must stay alive meanwhile."""
+        if damp is None:
+            lib().orc_record_decisions(self.h, None, None)
+            return
+        lib().orc_record_decisions(self.h, damp.ctypes.data_as(C.POINTER(C.c_int8)),
+                                   branch.ctypes.data_as(C.POINTER(C.c_int8)))
+
+    def bc(self):
+        mask = np.zeros(self.N, dtype=np.uint8)
+        val = np.zeros((self.N, 3))
+        _check(lib().orc_get_bc(self.h, mask.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(val)))
+        return mask, val
 
     def hydrostatic_ic(self):
         _check(lib().orc_hydrostatic_ic(self.h))

@@ -128,6 +128,12 @@ struct orc_context {
     double vmax;
     unsigned timeStep;
     int zero_done;
+    /* optional record of the discrete decisions of each step (test
+       infrastructure for the adjoint): sign(v) sign(f) of every node
+       component before damping [N][3], plastic branch of every tet [E][10]
+       (0 elastic, 1 shear, 2 tension) */
+    int8_t* rec_damp;
+    int8_t* rec_branch;
 };
 
 static __thread char g_err[256];
@@ -339,6 +345,19 @@ void orc_destroy(orc_context* c) {
 }
 
 void orc_set_bc_scan(orc_context* c, int faithful) { c->bc_scan = faithful; }
+
+void orc_record_decisions(orc_context* c, int8_t* damp, int8_t* branch) {
+    c->rec_damp = damp;
+    c->rec_branch = branch;
+}
+
+int orc_get_bc(orc_context* c, uint8_t* mask, double* value) {
+    for (unsigned n = 0; n < c->N; n++) {
+        mask[n] = c->bc_mask[n];
+        for (int d = 0; d < 3; d++) value[3 * (size_t)n + d] = c->bc_value[n][d];
+    }
+    return SNH_OK;
+}
 uint32_t orc_node_count(orc_context* c) { return c->N; }
 uint32_t orc_element_count(orc_context* c) { return c->E; }
 int orc_set_dt(orc_context* c, double dt) { c->dt = dt; return SNH_OK; }
@@ -807,7 +826,7 @@ static void soften(const snh_material* m, double ps, double* phi, double* psi, d
    plastic_BI (:159-223, hardening = 0) and viscoplastic_BI (:307-371). */
 static void mc_return(const snh_material* m, double (*st)[3], double s[3], double cn[3][3], double frictionAngle,
                       double dilationAngle, double cohesion, double tension_cutoff, double hardening, int with_hard,
-                      double* plasticStrain) {
+                      double* plasticStrain, int8_t* branch) {
     const double degrad = M_PI / 180.0;
     const double a1 = m->lambda + 2.0 * m->mu;
     const double a2 = m->lambda;
@@ -824,6 +843,7 @@ static void mc_return(const snh_material* m, double (*st)[3], double s[3], doubl
         double sP = tension_cutoff * anphi - 2 * cohesion * sqrt(anphi);
         double h = s[2] - tension_cutoff + aP * (s[0] - sP);
         ind = 1;
+        if (branch) *branch = h < 0.0 ? 1 : 2;
         if (h < 0.0) {
             if (with_hard)
                 alam = fs / (a1 - a2 * anpsi + a1 * anphi * anpsi - a2 * anphi + 2.0 * sqrt(anphi) * hardening);
@@ -848,6 +868,7 @@ static void mc_return(const snh_material* m, double (*st)[3], double s[3], doubl
         dep1 = 0.0;
         dep2 = 0.0;
         dep3 = 0.0;
+        if (branch) *branch = 0;
     }
     if (ind) {
         depm = (dep1 + dep2 + dep3) / 3.0;
@@ -905,7 +926,8 @@ static int constitutive_plastic(orc_context* c, unsigned e) {
             snprintf(g_err, sizeof g_err, "plastic: negative friction angle (elem %u tet %d)", e, t);
             return SNH_ENUMERIC;
         }
-        mc_return(m, st, s, cn, frictionAngle, dilationAngle, coh, tension_cutoff, 0.0, 0, &el->plasticStrain[t]);
+        mc_return(m, st, s, cn, frictionAngle, dilationAngle, coh, tension_cutoff, 0.0, 0, &el->plasticStrain[t],
+                  c->rec_branch ? &c->rec_branch[(size_t)e * NTET + t] : NULL);
         depls += el->plasticStrain[t] * el->tetra[t].volume;
         totalVolume += el->tetra[t].volume;
     }
@@ -1058,7 +1080,7 @@ static int constitutive_viscoplastic(orc_context* c, unsigned e) {
             return SNH_ENUMERIC;
         }
         mc_return(m, st, s, cn, frictionAngle, dilationAngle, coh, tension_cutoff, hardening, 1,
-                  &el->plasticStrain[t]);
+                  &el->plasticStrain[t], NULL);
         depls += el->plasticStrain[t] * el->tetra[t].volume;
         totalVolume += el->tetra[t].volume;
     }
@@ -1214,6 +1236,8 @@ int orc_force(orc_context* c, uint32_t n, double speedOfSound, double* mass, dou
 /* Snac_UpdateNodeMomentum_PreProcess, UpdateNode.c:74-158 (Cartesian branch) */
 int orc_update_node_preprocess(orc_context* c, uint32_t n, double inertialMass, double force[3]) {
     orc_node* nd = &c->node[n];
+    if (c->rec_damp)
+        for (int d = 0; d < 3; d++) c->rec_damp[3 * (size_t)n + d] = (int8_t)(GETSIGN(nd->velocity[d]) * GETSIGN(force[d]));
     force[0] -= c->demf * GETSIGN(nd->velocity[0]) * fabs(force[0]);
     force[1] -= c->demf * GETSIGN(nd->velocity[1]) * fabs(force[1]);
     force[2] -= c->demf * GETSIGN(nd->velocity[2]) * fabs(force[2]);

@@ -53,6 +53,12 @@ int  orc_get_max_threads(void);
 /* 1 = apply velocity BCs by the reference's linear scan of the BC index table
  * (VariableCondition.c:459-463); 0 (default) = O(1) per-node table. */
 void orc_set_bc_scan(orc_context* c, int faithful);
+/* record, during the following steps, sign(v) sign(f) of every node component
+   before damping ([N][3]) and the plastic branch of every tet ([E][10]:
+   0 elastic, 1 shear, 2 tension); NULL stops recording */
+void orc_record_decisions(orc_context* c, int8_t* damp, int8_t* branch);
+/* per-node velocity-BC mask (bit d = component d constrained) and values [N][3] */
+int orc_get_bc(orc_context* c, uint8_t* mask, double* value);
 
 int orc_set_element_material(orc_context* c, const uint8_t* mat, size_t nelem);
 int orc_set_element_field(orc_context* c, int field, const double* v, size_t nelem);

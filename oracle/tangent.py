@@ -1,0 +1,116 @@
+"""ctypes binding of the FP64 tangent-linear model (libtangent.so).
+
+TEST INFRASTRUCTURE ONLY: the checker of the GPU adjoint.  The product never
+imports it.
+
+record() runs the CPU oracle from the problem's initial state and keeps what
+the tangent model needs: the state after TimeStepZero, the velocity BCs, and
+per step dt, speed of sound, damping branch and plastic branch.  tangent()
+then runs the model with the parameter field moved along a direction and
+returns the final coordinates and their directional derivative.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from oracle.oracle import Oracle, build
+from snac_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libtangent.so")
+
+
+class TgProblem(C.Structure):
+    _fields_ = [("I", C.c_int32), ("J", C.c_int32), ("K", C.c_int32),
+                ("x0", C.c_void_p), ("v0", C.c_void_p), ("stress0", C.c_void_p), ("ps0", C.c_void_p),
+                ("mat", C.c_void_p), ("materials", C.c_void_p), ("nmat", C.c_int32),
+                ("refvisc", C.c_void_p), ("cohoff", C.c_void_p),
+                ("bc_mask", C.c_void_p), ("bc_value", C.c_void_p),
+                ("gravity", C.c_double), ("demf", C.c_double), ("param", C.c_int32)]
+
+
+class TgDecisions(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("dt", C.c_void_p), ("c", C.c_void_p),
+                ("damp", C.c_void_p), ("branch", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.tg_run.restype = C.c_int
+        L.tg_run.argtypes = [C.POINTER(TgProblem), C.POINTER(TgDecisions), C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def record(problem, nsteps, threads=4):
+    """Oracle run of nsteps with the decisions recorded."""
+    o = Oracle(problem, threads=threads)
+    if problem.hydrostatic:
+        o.hydrostatic_ic()
+    o.timestep_zero()
+    st0 = o.state()
+    mask, val = o.bc()
+    N, E = o.N, o.E
+    dts, cs = np.zeros(nsteps), np.zeros(nsteps)
+    damp = np.zeros((nsteps, N, 3), dtype=np.int8)
+    branch = np.zeros((nsteps, E, 10), dtype=np.int8)
+    for s in range(nsteps):
+        sc = o.scalars()
+        dts[s] = sc["dt"]
+        cs[s] = sc["speedOfSound"]
+        o.record_decisions(damp[s], branch[s])
+        o.step(1)
+        o.record_decisions(None, None)
+    final = o.state()
+    o.close()
+    return {"state0": st0, "bc_mask": mask, "bc_value": val, "dt": dts, "c": cs, "damp": damp,
+            "branch": branch, "final": final, "nsteps": nsteps}
+
+
+def tangent(problem, rec, param, direction=None, nsteps=None):
+    """(final coordinates [N, 3], derivative along `direction` [N, 3])."""
+    n = rec["nsteps"] if nsteps is None else nsteps
+    I, J, K = problem.nodes
+    cfg, keep = problem.to_c()
+    st = rec["state0"]
+    x0 = np.ascontiguousarray(st["coord"], dtype=np.float64)
+    v0 = np.ascontiguousarray(st["velocity"], dtype=np.float64)
+    s0 = np.ascontiguousarray(st["stress"], dtype=np.float64)
+    ps0 = np.ascontiguousarray(st["plastic_strain"], dtype=np.float64)
+    mat = None if problem.element_material is None else np.ascontiguousarray(problem.element_material, np.uint8)
+    rv = problem.fields.get(_abi.FIELD_REFVISC)
+    rv = None if rv is None else np.ascontiguousarray(rv, dtype=np.float64)
+    co = problem.fields.get(_abi.FIELD_COHESION_OFFSET)
+    co = None if co is None else np.ascontiguousarray(co, dtype=np.float64)
+    mask = np.ascontiguousarray(rec["bc_mask"])
+    val = np.ascontiguousarray(rec["bc_value"])
+    P = TgProblem(I, J, K, _ptr(x0), _ptr(v0), _ptr(s0), _ptr(ps0), _ptr(mat),
+                  C.cast(cfg.materials, C.c_void_p).value, cfg.nmaterials, _ptr(rv), _ptr(co),
+                  _ptr(mask), _ptr(val), cfg.gravity, cfg.demf, int(param))
+    dts = np.ascontiguousarray(rec["dt"])
+    cs = np.ascontiguousarray(rec["c"])
+    damp = np.ascontiguousarray(rec["damp"])
+    branch = np.ascontiguousarray(rec["branch"])
+    D = TgDecisions(n, _ptr(dts), _ptr(cs), _ptr(damp), _ptr(branch))
+    d = None if direction is None else np.ascontiguousarray(direction, dtype=np.float64)
+    x = np.zeros((problem.n_nodes, 3))
+    dx = np.zeros((problem.n_nodes, 3))
+    rc = lib().tg_run(C.byref(P), C.byref(D), _ptr(d), _ptr(x), _ptr(dx))
+    if rc != 0:
+        raise RuntimeError("tg_run failed")
+    del keep
+    return x, dx

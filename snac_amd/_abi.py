@@ -128,6 +128,19 @@ class TuneReport(C.Structure):
     _fields_ = [("elem_ms", C.c_double * 2), ("elem_variant", C.c_int32)]
 
 
+PARAM_LOG_VISCOSITY = 0
+PARAM_COHESION = 1
+
+
+class AdjointConfig(C.Structure):
+    _fields_ = [("param", C.c_int32), ("checkpoint_interval", C.c_int32)]
+
+
+class AdjointTimes(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("replay_ms", C.c_double), ("reverse_ms", C.c_double),
+                ("steps", C.c_int32), ("checkpoints", C.c_int32)]
+
+
 # Every function the header declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -154,6 +167,13 @@ SIGNATURES = {
     "snh_autotune": (_I, [_P, C.POINTER(TuneReport)]),
     "snh_set_elem_variant": (_I, [_P, C.c_int32]),
     "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
+    "snh_set_schedule": (_I, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int8)]),
+    "snh_adjoint_begin": (_I, [_P, C.POINTER(AdjointConfig)]),
+    "snh_adjoint_forward": (_I, [_P, C.c_int32]),
+    "snh_adjoint_reverse": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "snh_adjoint_get_times": (_I, [_P, C.POINTER(AdjointTimes)]),
+    "snh_adjoint_end": (_I, [_P]),
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 

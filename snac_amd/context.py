@@ -117,6 +117,57 @@ class Context:
         self._check(self.lib.snh_set_elem_variant(self.h, int(variant)))
         return self
 
+    def set_schedule(self, dt: np.ndarray, speed_of_sound: np.ndarray, damp: Optional[np.ndarray] = None):
+        """Prescribe dt, speed of sound and (optionally) the damping branch
+        ([n, N, 3] int8 codes sign(v) sign(f)) of the next n steps."""
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        cs = np.ascontiguousarray(speed_of_sound, dtype=np.float64)
+        n = dt.size
+        dp = None
+        if damp is not None:
+            damp = np.ascontiguousarray(damp, dtype=np.int8).reshape(n, self.N, 3)
+            dp = damp.ctypes.data_as(C.POINTER(C.c_int8))
+        self._check(self.lib.snh_set_schedule(self.h, n, _dp(dt), _dp(cs), dp))
+        return self
+
+    # -- adjoint ---------------------------------------------------------------
+    def adjoint_begin(self, param: int, checkpoint_interval: int = 50):
+        """Start recording for a reverse sweep with respect to a per-element
+        field (_abi.PARAM_LOG_VISCOSITY or _abi.PARAM_COHESION)."""
+        cfg = _abi.AdjointConfig(int(param), int(checkpoint_interval))
+        self._check(self.lib.snh_adjoint_begin(self.h, C.byref(cfg)))
+        return self
+
+    def adjoint_forward(self, nsteps: int):
+        """nsteps explicit steps (identical to step()) with checkpoints."""
+        self._check(self.lib.snh_adjoint_forward(self.h, int(nsteps)))
+        return self
+
+    def adjoint_reverse(self, coord_bar: Optional[np.ndarray] = None,
+                        velocity_bar: Optional[np.ndarray] = None) -> np.ndarray:
+        """d(misfit)/d(parameter field), [E], from d(misfit)/d(final coordinates)
+        and d(misfit)/d(final velocities), both [N, 3] (None = zero)."""
+        def arg(x):
+            if x is None:
+                return None, None
+            a = np.ascontiguousarray(x, dtype=np.float64).reshape(self.N, 3)
+            return a, _dp(a)
+        cb, cp = arg(coord_bar)
+        vb, vp = arg(velocity_bar)
+        grad = np.zeros(self.E)
+        self._check(self.lib.snh_adjoint_reverse(self.h, cp, vp, _dp(grad)))
+        return grad
+
+    def adjoint_times(self) -> Dict[str, float]:
+        t = _abi.AdjointTimes()
+        self._check(self.lib.snh_adjoint_get_times(self.h, C.byref(t)))
+        return {"forward_ms": t.forward_ms, "replay_ms": t.replay_ms, "reverse_ms": t.reverse_ms,
+                "steps": t.steps, "checkpoints": t.checkpoints}
+
+    def adjoint_end(self):
+        self._check(self.lib.snh_adjoint_end(self.h))
+        return self
+
     def profile(self, enable: bool = True):
         self._check(self.lib.snh_profile(self.h, int(enable)))
 

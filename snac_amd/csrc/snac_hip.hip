@@ -403,6 +403,29 @@ __device__ __forceinline__ double tan_phi(const DevMaterial& m, double phi) {
     return (phi == m.fa[0]) ? m.tanphi : tan(phi * degrad);
 }
 
+/* True when the trial stress st (00,11,22,01,02,12) cannot reach the
+   Mohr-Coulomb or tension surface.  The principal stresses of a symmetric
+   3x3 lie within p +- sqrt(2/3) |dev st| (p the mean stress; equality for
+   an axisymmetric deviator), which bounds fs = s0 - s2 anphi + 2 c sqrt(anphi)
+   from below and ft = s2 - tc from above.  The margin covers the eigen-
+   solver's error, including the reference's int abs() that drops entries
+   below 1 Pa.  For such a tet mc_return changes neither the stress nor the
+   plastic strain, so its eigen-decomposition is skipped: the results are
+   bit-identical. */
+__device__ __forceinline__ bool mc_surely_elastic(const double st[6], double sphi, double cohesion, double tc) {
+    const double p = (st[0] + st[1] + st[2]) * (1.0 / 3.0);
+    const double d0 = st[0] - p, d1 = st[1] - p, d2 = st[2] - p;
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (st[3] * st[3] + st[4] * st[4] + st[5] * st[5]);
+    const double r = sqrt(r2 * (2.0 / 3.0));
+    const double anphi = (1.0 + sphi) / (1.0 - sphi);
+    const double smin = p - r, smax = p + r;
+    const double scale = fabs(p) + r + fabs(cohesion) + fabs(tc);
+    const double margin = 64.0 + 1e-9 * scale * (1.0 + anphi);
+    const double fs_lo = smin - smax * anphi + 2.0 * cohesion * sqrt(anphi);
+    const double ft_hi = smax - tc;
+    return fs_lo > margin && ft_hi < -margin;
+}
+
 /* viscosity of maxwell (:123-151) / viscoplastic_BI (:174-218); the
    temperature field is zero (SnacTemperature not on the path). */
 __device__ __forceinline__ double viscosity_of(const DevMaterial& m, double rvisc, double sd0, double sd1, double sd2,
@@ -515,7 +538,7 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
     const bool in_wave = lane < EPW * NTET;
     const int le = in_wave ? lane / NTET : 0;
     const int t = in_wave ? lane - le * NTET : 0;
-    const int e = blockIdx.x * EPW + le;
+    const int e = xcd_block() * EPW + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
@@ -633,19 +656,21 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
         S[0][1] = S[1][0] = st[3];
         S[0][2] = S[2][0] = st[4];
         S[1][2] = S[2][1] = st[5];
-        double d3[3], cn[3][3];
-        principal_stresses(S, d3, cn);
         const double coh = a.cohoff ? sp.coh + a.cohoff[ec] : sp.coh;
         double tc = m.ten_off;
         if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
         double sphi, spsi, dps = 0.0;
         angle_terms(m, sp.phi, sp.psi, sphi, spsi);
-        if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
-            ps = ps + dps;
-            st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-            st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+        if (!mc_surely_elastic(st, sphi, coh, tc)) {
+            double d3[3], cn[3][3];
+            principal_stresses(S, d3, cn);
+            if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
+                ps = ps + dps;
+                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            }
         }
-        if (active) a.ps[(size_t)t * E + ec] = ps;
+        if (active) a.ps_out[(size_t)t * E + ec] = ps;
     } else if (is_vp) {
         double trace_strain = sn[0] + sn[1] + sn[2];
         double d0, d1, d2;
@@ -668,7 +693,7 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
             st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
             st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
         }
-        if (active) a.ps[(size_t)t * E + ec] = ps;
+        if (active) a.ps_out[(size_t)t * E + ec] = ps;
     } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
         /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
         double Xp[4][3];
@@ -712,7 +737,7 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
     st[2] += -1.0 * tr + avgTrace;
     if (active) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + ec] = st[c];
+        for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + ec] = st[c];
     }
 }
 
@@ -724,7 +749,7 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
  * ------------------------------------------------------------------------ */
 template <int RSET>
 __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
     const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
     const double dt = a.scal->dt;
@@ -806,8 +831,6 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             S[0][1] = S[1][0] = st[3];
             S[0][2] = S[2][0] = st[4];
             S[1][2] = S[2][1] = st[5];
-            double d3[3], cn[3][3];
-            principal_stresses(S, d3, cn);
             soften(m, plasticStrain, pphi, ppsi, pcoh, phard);
             double coh = a.cohoff ? pcoh + cohoff : pcoh;
             if (pphi >= 0.0) {
@@ -818,7 +841,6 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
                    negative friction angle otherwise is an error the host
                    checks before the run */
                 plasticStrain = 0.0;
-                a.ps[(size_t)t * E + e] = 0.0;
                 pphi = m.fa[0];
                 ppsi = m.da[0];
                 pcoh = m.coh[0];
@@ -827,11 +849,18 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             double sphi, spsi;
             angle_terms(m, pphi, ppsi, sphi, spsi);
             double dps = 0.0;
-            if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, 0.0, false, dps)) {
-                a.ps[(size_t)t * E + e] = plasticStrain + dps;
-                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            if (!mc_surely_elastic(st, sphi, coh, ptc)) {
+                double d3[3], cn[3][3];
+                principal_stresses(S, d3, cn);
+                if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, 0.0, false, dps)) {
+                    plasticStrain = plasticStrain + dps;
+                    st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                    st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+                }
             }
+            /* in place, an unchanged value need not be stored */
+            if (a.ps_out != a.ps || plasticStrain != a.ps[(size_t)t * E + e])
+                a.ps_out[(size_t)t * E + e] = plasticStrain;
         } else if ((RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)) {
             double plasticStrain = a.ps[(size_t)t * E + e];
             double trace_strain = sn[0] + sn[1] + sn[2];
@@ -852,7 +881,6 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
                 if (pphi > 0.0 && ptc == 0.0) ptc = coh / tan_phi(m, pphi);
             } else {
                 plasticStrain = 0.0;
-                a.ps[(size_t)t * E + e] = 0.0;
                 pphi = m.fa[0];
                 ppsi = m.da[0];
                 pcoh = m.coh[0];
@@ -862,10 +890,13 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             angle_terms(m, pphi, ppsi, sphi, spsi);
             double dps = 0.0;
             if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, phard, true, dps)) {
-                a.ps[(size_t)t * E + e] = plasticStrain + dps;
+                plasticStrain = plasticStrain + dps;
                 st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
                 st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
             }
+            /* in place, an unchanged value need not be stored */
+            if (a.ps_out != a.ps || plasticStrain != a.ps[(size_t)t * E + e])
+                a.ps_out[(size_t)t * E + e] = plasticStrain;
         } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
             /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
             double Xp[4][3];
@@ -914,7 +945,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
         sig[t][1] += -1.0 * tr[t] + avgTrace;
         sig[t][2] += -1.0 * tr[t] + avgTrace;
 #pragma unroll
-        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + e] = sig[t][c];
+        for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + e] = sig[t][c];
     }
 }
 
@@ -971,7 +1002,7 @@ __global__ void k_volumes(StepArgs a, double* __restrict__ out) {
 
 __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcd_block() * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
     if (n < a.N) {
         const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
@@ -1034,9 +1065,28 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
         /* damping, velocity, BC, coordinates (UpdateNode.c:60-158) */
         const size_t N = (size_t)a.N;
         double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
-        f0 -= a.demf * GETSIGN(v0) * fabs(f0);
-        f1 -= a.demf * GETSIGN(v1) * fabs(f1);
-        f2 -= a.demf * GETSIGN(v2) * fabs(f2);
+        if (a.tape_f) {
+            /* what the reverse pass needs: the undamped force, the mass and
+               the damping branch of each component */
+            a.tape_f[n] = f0;
+            a.tape_f[N + n] = f1;
+            a.tape_f[2 * N + n] = f2;
+            a.tape_m[n] = im;
+            const int c0 = (int)(GETSIGN(v0) * GETSIGN(f0)) + 1;
+            const int c1 = (int)(GETSIGN(v1) * GETSIGN(f1)) + 1;
+            const int c2 = (int)(GETSIGN(v2) * GETSIGN(f2)) + 1;
+            a.tape_code[n] = (uint8_t)(c0 | (c1 << 2) | (c2 << 4));
+        }
+        double sv0 = GETSIGN(v0), sv1 = GETSIGN(v1), sv2 = GETSIGN(v2);
+        if (a.damp_sched) {
+            const uint8_t ds = a.damp_sched[n];
+            sv0 = (double)((int)(ds & 3u) - 1) * GETSIGN(f0);
+            sv1 = (double)((int)((ds >> 2) & 3u) - 1) * GETSIGN(f1);
+            sv2 = (double)((int)((ds >> 4) & 3u) - 1) * GETSIGN(f2);
+        }
+        f0 -= a.demf * sv0 * fabs(f0);
+        f1 -= a.demf * sv1 * fabs(f1);
+        f2 -= a.demf * sv2 * fabs(f2);
         v0 += dt * f0 / im;
         v1 += dt * f1 / im;
         v2 += dt * f2 / im;
@@ -1048,9 +1098,9 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
             if (b.mask & 4u) v2 = b.val[2];
             vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
         }
-        a.v[n] = v0;
-        a.v[N + n] = v1;
-        a.v[2 * N + n] = v2;
+        a.v_out[n] = v0;
+        a.v_out[N + n] = v1;
+        a.v_out[2 * N + n] = v2;
         a.xn[n] = a.xc[n] + dt * v0;
         a.xn[N + n] = a.xc[N + n] + dt * v1;
         a.xn[2 * N + n] = a.xc[2 * N + n] + dt * v2;
@@ -1080,7 +1130,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_vmax(StepArgs a) {
  * ------------------------------------------------------------------------ */
 __global__ void __launch_bounds__(GEO_BLOCK) k_geo(StepArgs a) {
     __shared__ double sh[GEO_BLOCK / 64];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     double emin = INFINITY;
     if (e < a.E) {
         const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
@@ -1149,6 +1199,17 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvm
     }
 }
 
+#include "snh_adjoint.h"
+
+/* prescribed dt and speed of sound of one step (snh_set_schedule) */
+__global__ void k_set_scalars(DevScalars* __restrict__ s, const double* __restrict__ dt,
+                              const double* __restrict__ cs) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s->dt = *dt;
+        s->speedOfSound = *cs;
+    }
+}
+
 /* ==========================================================================
  * host side
  * ======================================================================== */
@@ -1200,6 +1261,14 @@ struct snh_context {
     bool needs_visc;
     uint32_t timeStep;
     int elem_variant;   /* SNH_ELEM_* */
+    struct AdjointCtx* adj;   /* recorded adjoint run, or null */
+    /* prescribed dt / speed of sound / damping branch for steps
+       [sched_base, sched_base + sched_n) of timeStep */
+    double* sched_dt;
+    double* sched_c;
+    uint8_t* sched_damp;      /* [sched_n][N] or null */
+    uint32_t sched_base;
+    int32_t sched_n;
     /* kernel timing */
     bool profiling;
     std::vector<hipEvent_t> ev_pool;
@@ -1292,8 +1361,15 @@ static StepArgs make_args(snh_context* c) {
     a.xp = c->x[c->cur ^ 1];
     a.xn = c->x[c->cur ^ 1];
     a.v = c->v;
+    a.v_out = c->v;
     a.stress = c->stress;
+    a.stress_out = c->stress;
     a.ps = c->ps;
+    a.ps_out = c->ps;
+    a.tape_f = nullptr;
+    a.tape_m = nullptr;
+    a.tape_code = nullptr;
+    a.damp_sched = nullptr;
     a.visc = c->visc;
     a.mat = c->mat;
     a.refvisc = c->refvisc;
@@ -1314,7 +1390,7 @@ static StepArgs make_args(snh_context* c) {
 
 static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
-                    c->bc, c->part_min, c->part_vmax, c->scal, c->tmp};
+                    c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1501,6 +1577,11 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     }
     c->cur = 0;
     c->elem_variant = SNH_ELEM_TET_PER_LANE;
+    c->adj = nullptr;
+    c->sched_dt = c->sched_c = nullptr;
+    c->sched_damp = nullptr;
+    c->sched_base = 0;
+    c->sched_n = 0;
     if (const char* ev = getenv("SNH_ELEM_VARIANT")) c->elem_variant = atoi(ev) == 1 ? 1 : 0;
     c->zero_done = false;
     c->rotation_pending = false;
@@ -1509,10 +1590,13 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     return SNH_OK;
 }
 
+static void adj_free(struct AdjointCtx* A);
+
 extern "C" int snh_context_destroy(snh_context* c) {
     if (!c) return SNH_OK;
     (void)hipStreamSynchronize(c->stream);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    adj_free(c->adj);
     free_all(c);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1661,9 +1745,17 @@ static int launch_elem_args(snh_context* c, StepArgs a) {
     return SNH_OK;
 }
 
+/* index of timeStep t in the prescribed schedule, or -1 */
+static int sched_index(const snh_context* c, uint32_t t) {
+    if (c->sched_n <= 0 || t < c->sched_base || t - c->sched_base >= (uint32_t)c->sched_n) return -1;
+    return (int)(t - c->sched_base);
+}
+
 static int launch_node(snh_context* c) {
     KTimer kt(c, KK_NODE);
     StepArgs a = make_args(c);
+    const int si = sched_index(c, c->timeStep);
+    if (si >= 0 && c->sched_damp) a.damp_sched = c->sched_damp + (size_t)si * c->N;
     if (c->cfg.precision == SNH_PRECISION_MIXED)
         hipLaunchKernelGGL(k_node_mx, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     else
@@ -1767,6 +1859,12 @@ extern "C" int snh_step(snh_context* c, int nsteps) {
     if (nsteps < 0) return set_err(SNH_EINVAL, "negative step count");
     for (int s = 0; s < nsteps; s++) {
         int rc;
+        const int si = sched_index(c, c->timeStep);
+        if (si >= 0) {
+            hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c->stream, c->scal, c->sched_dt + si,
+                               c->sched_c + si);
+            HIPCHK(hipGetLastError());
+        }
         if ((rc = launch_elem(c))) return rc;
         if ((rc = launch_node(c))) return rc;
         if ((rc = launch_geo(c))) return rc;
@@ -1774,6 +1872,44 @@ extern "C" int snh_step(snh_context* c, int nsteps) {
         c->timeStep++;
     }
     c->rotation_pending = c->rotation_pending || nsteps > 0;
+    return SNH_OK;
+}
+
+extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt, const double* cs,
+                                const int8_t* damp) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (nsteps < 0 || (nsteps > 0 && (!dt || !cs))) return set_err(SNH_EINVAL, "bad schedule");
+    if (nsteps > 0 && c->cfg.precision != SNH_PRECISION_FP64)
+        return set_err(SNH_EUNSUPPORTED, "schedules apply to the FP64 path");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    void* old[] = {c->sched_dt, c->sched_c, c->sched_damp};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    c->sched_dt = c->sched_c = nullptr;
+    c->sched_damp = nullptr;
+    c->sched_n = 0;
+    if (nsteps == 0) return SNH_OK;
+    const size_t N = c->N;
+    HIPCHK(hipMalloc((void**)&c->sched_dt, nsteps * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c->sched_c, nsteps * sizeof(double)));
+    HIPCHK(hipMemcpy(c->sched_dt, dt, nsteps * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->sched_c, cs, nsteps * sizeof(double), hipMemcpyHostToDevice));
+    if (damp) {
+        std::vector<uint8_t> packed((size_t)nsteps * N);
+        for (size_t i = 0; i < packed.size(); i++) {
+            int code = 0;
+            for (int d = 0; d < 3; d++) {
+                const int v = damp[3 * i + d];
+                if (v < -1 || v > 1) return set_err(SNH_EINVAL, "damping codes must be -1, 0 or 1");
+                code |= (v + 1) << (2 * d);
+            }
+            packed[i] = (uint8_t)code;
+        }
+        HIPCHK(hipMalloc((void**)&c->sched_damp, packed.size()));
+        HIPCHK(hipMemcpy(c->sched_damp, packed.data(), packed.size(), hipMemcpyHostToDevice));
+    }
+    c->sched_base = c->timeStep;
+    c->sched_n = nsteps;
     return SNH_OK;
 }
 
@@ -1910,29 +2046,42 @@ extern "C" int snh_get_kernel_times(snh_context* c, snh_kernel_times* out) {
     return SNH_OK;
 }
 
-__global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) dst[i] = src[i];
+/* plain 16-byte copy, 4 loads in flight per lane, one pass over the buffer */
+constexpr int COPY_BLOCK = 256, COPY_UNROLL = 4;
+typedef double copy_vec __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(COPY_BLOCK) k_copy(const copy_vec* __restrict__ src, copy_vec* __restrict__ dst,
+                                                     size_t n) {
+    const size_t base = (size_t)blockIdx.x * (COPY_BLOCK * COPY_UNROLL) + threadIdx.x;
+    copy_vec r[COPY_UNROLL];
+#pragma unroll
+    for (int u = 0; u < COPY_UNROLL; u++) {
+        size_t i = base + (size_t)u * COPY_BLOCK;
+        if (i < n) r[u] = __builtin_nontemporal_load(&src[i]);
+    }
+#pragma unroll
+    for (int u = 0; u < COPY_UNROLL; u++) {
+        size_t i = base + (size_t)u * COPY_BLOCK;
+        if (i < n) __builtin_nontemporal_store(r[u], &dst[i]);
+    }
 }
 
 extern "C" int snh_measure_copy_bandwidth(size_t nbytes, int iters, double* gbps) {
     if (!gbps || nbytes < 1024 || iters < 1) return set_err(SNH_EINVAL, "bad arguments");
-    size_t n = nbytes / sizeof(double2);
-    double2 *a = nullptr, *b = nullptr;
-    HIPCHK(hipMalloc((void**)&a, n * sizeof(double2)));
-    if (hipMalloc((void**)&b, n * sizeof(double2)) != hipSuccess) {
+    size_t n = nbytes / sizeof(copy_vec);
+    copy_vec *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void**)&a, n * sizeof(copy_vec)));
+    if (hipMalloc((void**)&b, n * sizeof(copy_vec)) != hipSuccess) {
         (void)hipFree(a);
         return set_err(SNH_ENOMEM, "copy probe allocation failed");
     }
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    (void)hipMemset(a, 0, n * sizeof(double2));
-    int blocks = 256 * 8 * 4;
-    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipMemset(a, 0, n * sizeof(copy_vec));
+    const unsigned blocks = (unsigned)((n + COPY_BLOCK * COPY_UNROLL - 1) / (COPY_BLOCK * COPY_UNROLL));
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(COPY_BLOCK), 0, 0, a, b, n);
     (void)hipEventRecord(e0, 0);
-    for (int it = 0; it < iters; it++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    for (int it = 0; it < iters; it++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(COPY_BLOCK), 0, 0, a, b, n);
     (void)hipEventRecord(e1, 0);
     hipError_t err = hipEventSynchronize(e1);
     float ms = 0.0f;
@@ -1942,7 +2091,7 @@ extern "C" int snh_measure_copy_bandwidth(size_t nbytes, int iters, double* gbps
     (void)hipFree(a);
     (void)hipFree(b);
     if (err != hipSuccess) return set_err(SNH_EHIP, "copy probe failed: %s", hipGetErrorString(err));
-    *gbps = 2.0 * (double)(n * sizeof(double2)) * iters / (ms * 1e-3) / 1e9;
+    *gbps = 2.0 * (double)(n * sizeof(copy_vec)) * iters / (ms * 1e-3) / 1e9;
     return SNH_OK;
 }
 
@@ -1980,7 +2129,9 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
         c->elem_variant = v;
         StepArgs a = make_args(c);
         a.stress = st;
+        a.stress_out = st;
         a.ps = ps;
+        a.ps_out = ps;
         a.visc = visc;
         float best = 1e30f;
         for (int r = 0; r < reps + 1 && rc == SNH_OK; r++) {
@@ -2015,3 +2166,5 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
     if (out) *out = rep;
     return SNH_OK;
 }
+
+#include "snh_adjoint_host.h"

@@ -76,14 +76,30 @@ struct StepArgs {
     const double* __restrict__ xc;  /* coordinates of this step [3][N]     */
     const double* __restrict__ xp;  /* coordinates of the step before      */
     double* __restrict__ xn;        /* coordinates written by the node pass */
-    double* __restrict__ v;         /* velocity [3][N]                     */
-    double* __restrict__ stress;    /* [6][10][E]                          */
-    double* __restrict__ ps;        /* [10][E] or null                     */
+    /* stress, plastic strain and velocity are read from the first pointer
+       and written through the second; the normal step passes the same
+       buffer twice, a recorded replay passes consecutive tape slots */
+    const double* v;                /* velocity [3][N]                     */
+    double* v_out;
+    const double* stress;           /* [6][10][E]                          */
+    double* stress_out;
+    const double* ps;               /* [10][E] or null                     */
+    double* ps_out;
     double* __restrict__ visc;      /* [10][E] or null                     */
     const uint8_t* __restrict__ mat;      /* [E] or null (material 0)      */
     const double* __restrict__ refvisc;   /* [E] or null                   */
     const double* __restrict__ cohoff;    /* [E] or null                   */
     const uint8_t* __restrict__ bc;       /* [N] BC code, 0 = none         */
+    /* replay tape of the node pass (null when not recording): force before
+       damping [3][N], inertial mass [N], sign(v)*sign(f) per component packed
+       2 bits each [N] */
+    double* tape_f;
+    double* tape_m;
+    uint8_t* tape_code;
+    /* prescribed damping branch of this step (packed as tape_code), or null:
+       sign(v) is replaced by code * sign(f), which gives the same bits when
+       the code is the one the step would compute */
+    const uint8_t* damp_sched;
     double* __restrict__ part_min;
     double* __restrict__ part_vmax;
     DevScalars* __restrict__ scal;
@@ -109,6 +125,15 @@ SNH_HD void face_cross(const double* c1, const double* c2, const double* c3, dou
     a = y2 * z3 - z2 * y3;
     b = z2 * x3 - x2 * z3;
     c = x2 * y3 - y2 * x3;
+}
+
+/* Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD
+   b % 8), each with its own L2.  Renumber so that every XCD works through one
+   contiguous range of the mesh: elements sharing nodes then share an L2. */
+__device__ __forceinline__ int xcd_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + i;
 }
 
 /* Tetrahedra_Volume (Tetrahedra.c:44-67).  Its cross product is the one of

@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
     const bool in_wave = lane < EPW * NTET;
     const int le = in_wave ? lane / NTET : 0;
     const int t = in_wave ? lane - le * NTET : 0;
-    const int e = blockIdx.x * EPW + le;
+    const int e = xcd_block() * EPW + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
@@ -224,19 +224,21 @@ __global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
         S[0][1] = S[1][0] = st[3];
         S[0][2] = S[2][0] = st[4];
         S[1][2] = S[2][1] = st[5];
-        double d3[3], cn[3][3];
-        principal_stresses(S, d3, cn);
         const double coh = a.cohoff ? sp.coh + a.cohoff[ec] : sp.coh;
         double tc = m.ten_off;
         if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
         double sphi, spsi, dps = 0.0;
         angle_terms(m, sp.phi, sp.psi, sphi, spsi);
-        if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
-            ps = ps + dps;
-            st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-            st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+        if (!mc_surely_elastic(st, sphi, coh, tc)) {
+            double d3[3], cn[3][3];
+            principal_stresses(S, d3, cn);
+            if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
+                ps = ps + dps;
+                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+            }
         }
-        if (active) a.ps[(size_t)t * E + ec] = ps;
+        if (active) a.ps_out[(size_t)t * E + ec] = ps;
     } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
         /* volumetric strain from FP64 volumes of the two configurations */
         double Xc[4][3], Xp[4][3];
@@ -300,14 +302,14 @@ __global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
     st[2] += -tr + avgTrace;
     if (active) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) a.stress[((size_t)c * NTET + t) * E + ec] = st[c];
+        for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + ec] = st[c];
     }
 }
 
 /* node gather, mixed precision */
 __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcd_block() * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
     if (n < a.N) {
         const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
@@ -323,25 +325,16 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
             if (ei < 0 || ej < 0 || ek < 0 || ei >= a.EI || ej >= a.EJ || ek >= a.EK) continue;
             const int e = ei + a.EI * (ej + a.EJ * ek);
             const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
-            const float idens = (float)(m.lam2mu / (c * c));
-            const float rho = (float)m.rho_eff;
-            const float grav8 = (float)a.gravity * rho * 0.125f;
             double X[8][3];
             load_hex(a, a.xc, ei, ej, ek, X);
-            float Ef[8][3];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                Ef[q][0] = (float)(X[q][0] - xn0);
-                Ef[q][1] = (float)(X[q][1] - xn1);
-                Ef[q][2] = (float)(X[q][2] - xn2);
-            }
             /* reference stress of this element: its first listed tet */
             const int tref = kNodeToTetra[slot][0];
             double sref[6];
 #pragma unroll
             for (int q = 0; q < 6; q++) sref[q] = a.stress[((size_t)q * NTET + tref) * E + e];
             double C0 = 0.0, C1 = 0.0, C2 = 0.0;   /* sum of FP64 face vectors (2 n A) */
-            float g0 = 0.f, g1 = 0.f, g2 = 0.f, mi = 0.f, gr = 0.f;
+            double vs = 0.0;                         /* sum of the 5 tet volumes, FP64 */
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f;
 #pragma unroll
             for (int idx = 0; idx < 5; idx++) {
 #pragma clang fp contract(fast)
@@ -350,36 +343,35 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
                 const int q0 = kTetraToNode[t][kFaceToNode[f][0]];
                 const int q1 = kTetraToNode[t][kFaceToNode[f][1]];
                 const int q2 = kTetraToNode[t][kFaceToNode[f][2]];
+                /* the tet's fourth node, opposite face f */
+                const int qo = kTetraToNode[t][6 - kFaceToNode[f][0] - kFaceToNode[f][1] - kFaceToNode[f][2]];
                 double dax, day, daz;
                 face_cross(X[q0], X[q1], X[q2], dax, day, daz);
                 C0 += dax;
                 C1 += day;
                 C2 += daz;
-                /* FP32 geometry of the tet */
-                const float* p0 = Ef[kTetraToNode[t][0]];
-                const float* p1 = Ef[kTetraToNode[t][1]];
-                const float* p2 = Ef[kTetraToNode[t][2]];
-                const float* p3 = Ef[kTetraToNode[t][3]];
-                float x2 = p1[0] - p0[0], y2 = p1[1] - p0[1], z2 = p1[2] - p0[2];
-                float x3 = p2[0] - p0[0], y3 = p2[1] - p0[1], z3 = p2[2] - p0[2];
-                float x4 = p3[0] - p0[0], y4 = p3[1] - p0[1], z4 = p3[2] - p0[2];
-                const float V = fabsf((x4 * (y2 * z3 - z2 * y3) + y4 * (z2 * x3 - x2 * z3) + z4 * (x2 * y3 - x3 * y2)) *
-                                      (1.0f / 6.0f));
+                /* tet volume from the same face: |(x_o - x_0) . (2 n A)| / 6.
+                   FP64: the gravity load it scales is balanced by the
+                   FP64 pre-stress term, so its rounding error is not
+                   damped by the force balance. */
+                vs += fabs((X[qo][0] - X[q0][0]) * dax + (X[qo][1] - X[q0][1]) * day +
+                           (X[qo][2] - X[q0][2]) * daz);
                 const float ax = (float)dax, ay = (float)day, az = (float)daz;   /* 2 n A */
                 float ds[6];
 #pragma unroll
                 for (int q = 0; q < 6; q++) ds[q] = (float)(a.stress[((size_t)q * NTET + t) * E + e] - sref[q]);
-                mi += 0.25f * idens * V;
                 g0 += ds[0] * ax + ds[3] * ay + ds[4] * az;
                 g1 += ds[3] * ax + ds[1] * ay + ds[5] * az;
                 g2 += ds[4] * ax + ds[5] * ay + ds[2] * az;
-                gr += grav8 * V;
             }
+            vs *= 1.0 / 6.0;
+            const double gr = a.gravity * m.rho_eff * 0.125 * vs;
+            const double mi = 0.25 * (m.lam2mu / (c * c)) * vs;
             /* sigma . n A / 6 = sigma . (2 n A) / 12 */
             f0 += (sref[0] * C0 + sref[3] * C1 + sref[4] * C2) / 12.0 + (double)g0 / 12.0;
-            f1 += (sref[3] * C0 + sref[1] * C1 + sref[5] * C2) / 12.0 + (double)g1 / 12.0 - (double)gr;
+            f1 += (sref[3] * C0 + sref[1] * C1 + sref[5] * C2) / 12.0 + (double)g1 / 12.0 - gr;
             f2 += (sref[4] * C0 + sref[5] * C1 + sref[2] * C2) / 12.0 + (double)g2 / 12.0;
-            if (a.dtType == SNH_DT_DYNAMIC) im += (double)mi;
+            if (a.dtType == SNH_DT_DYNAMIC) im += mi;
         }
         if (a.dtType != SNH_DT_DYNAMIC) im = 1.0; /* host rejects other dt types in mixed mode */
         double v0 = a.v[n], v1 = a.v[NN + n], v2 = a.v[2 * NN + n];
@@ -397,9 +389,9 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
             if (b.mask & 4u) v2 = b.val[2];
             vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
         }
-        a.v[n] = v0;
-        a.v[NN + n] = v1;
-        a.v[2 * NN + n] = v2;
+        a.v_out[n] = v0;
+        a.v_out[NN + n] = v1;
+        a.v_out[2 * NN + n] = v2;
         a.xn[n] = xn0 + dt * v0;
         a.xn[NN + n] = xn1 + dt * v1;
         a.xn[2 * NN + n] = xn2 + dt * v2;

@@ -275,8 +275,32 @@ def maxwell_viscoelastic(elements=(192, 48, 192), extent_km=(300.0, 75.0, 300.0)
     return p
 
 
+def mohr_coulomb_shear(elements=(64, 32, 64), extent_km=(64.0, 32.0, 64.0), seed=1) -> Problem:
+    """Simple shear of a Mohr-Coulomb box without gravity: the top moves in x
+    at 1 cm/yr over a fixed base, z-faces free-slip.  Cohesion softens from
+    1 MPa to 0.2 MPa over plastic strain 0..0.05 (friction 30 deg, dilation
+    0), strain_inert 1e-4: the box yields within a few steps, with distinct
+    principal stresses (the return is then a smooth function of the trial
+    stress).  A seeded 2x2x2 weak seed sits in the lower half."""
+    p = mohr_coulomb_extension(elements=elements, extent_km=extent_km, seed=seed)
+    m = p.materials[0]
+    m.cohesion = (1.0e6, 2.0e5, 2.0e5)
+    m.plstrain = (0.0, 0.05, 1.0e3)
+    p.gravity = 0.0
+    p.hydrostatic = False
+    p.strain_inert = 1.0e-4
+    v = CM_PER_YEAR
+    p.bcs = [("top", _abi.VX, (v, 0.0, 0.0)),
+             ("bottom", _abi.VX | _abi.VY | _abi.VZ, (0.0, 0.0, 0.0)),
+             ("front", _abi.VZ, (0.0, 0.0, 0.0)),
+             ("back", _abi.VZ, (0.0, 0.0, 0.0))]
+    p.name = "mohr_coulomb_shear"
+    return p
+
+
 WORKLOADS = {
     "elastic_box_compression": elastic_box_compression,
     "mohr_coulomb_extension": mohr_coulomb_extension,
     "maxwell_viscoelastic": maxwell_viscoelastic,
+    "mohr_coulomb_shear": mohr_coulomb_shear,
 }

@@ -47,6 +47,18 @@ def test_mohr_coulomb_extension_small(variant):
 
 
 @VARIANTS
+def test_mohr_coulomb_shear_yielding(variant):
+    """Simple shear without gravity: most tets yield from the fourth step on,
+    in the shear and tension branches, with the eigen-decomposition skipped
+    only where the yield functions provably keep their sign."""
+    p = P.mohr_coulomb_shear(elements=(8, 5, 6))
+    ref, got, rsc, gsc = run_both(p, 40, variant=variant)
+    _check(f"mc_shear[{variant}]", ref, got, rsc, gsc)
+    ps = ref["plastic_strain"]
+    assert np.sum((ps > 0.0) & (ps != 0.5)) > 100   # tets that yielded during the run
+
+
+@VARIANTS
 def test_maxwell_small(variant):
     p = P.maxwell_viscoelastic(elements=(10, 4, 10))
     ref, got, rsc, gsc = run_both(p, 60, variant=variant)
