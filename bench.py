@@ -68,7 +68,7 @@ def cpu_baseline(problem, seconds, threads, max_steps=200):
     return problem.n_elements * steps / (t1 - t0), t1 - t0, steps
 
 
-def adjoint_bench(make, param, steps, interval):
+def adjoint_bench(make, param, steps, interval, flags=0):
     """Forward time of `steps` plain steps vs the adjoint's reverse sweep
     (segment replays + reverse kernels) on the same inputs."""
     import numpy as np
@@ -89,7 +89,7 @@ def adjoint_bench(make, param, steps, interval):
     c = Context(p).Construct().Build().Initialise()
     top = top_nodes(p)
     x0 = c.state(fields=("coord",))["coord"]
-    c.adjoint_begin(param, interval)
+    c.adjoint_begin(param, interval, flags)
     c.adjoint_forward(steps)
     x = c.state(fields=("coord",))["coord"]
     cb = np.zeros((p.n_nodes, 3))
@@ -107,7 +107,39 @@ def adjoint_bench(make, param, steps, interval):
         "ratio": (t["replay_ms"] + t["reverse_ms"]) / plain_ms,
         "ratio_reverse_kernels": t["reverse_ms"] / plain_ms,
         "grad_norm": float(np.linalg.norm(g)),
+        "flags": flags,
     }
+
+
+def ensemble_bench(n_models, steps, graph_steps=10):
+    """Aggregate element-steps/s of an ensemble of uneven elasto-plastic
+    models: one graph launch per graph_steps steps (members on parallel
+    branches) against the same members stepped one after the other."""
+    from snac_amd import Context
+    from snac_amd.ensemble import Ensemble, elastoplastic_ensemble
+    probs = elastoplastic_ensemble(n_models=n_models)
+    total_e = sum(p.n_elements for p in probs)
+    members = [Context(p).Construct().Build().Initialise() for p in probs]
+    ens = Ensemble(members, graph_steps=graph_steps, autotune=True)
+    variants = [t["elem_variant"] for t in ens.tuning]
+    ens.step(graph_steps).synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps).synchronize()
+    t_ens = time.perf_counter() - t0
+    ens.destroy()
+    t0 = time.perf_counter()
+    for m in members:
+        m.step(steps)
+    for m in members:
+        m.synchronize()
+    t_seq = time.perf_counter() - t0
+    for m in members:
+        m.Destroy()
+    return {"models": n_models, "elements_total": total_e, "steps": steps, "graph_steps": graph_steps,
+            "value": total_e * steps / t_ens, "unit": "element-steps/s",
+            "sequential_value": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_ens,
+            "elem_variants": {"tet_per_lane": variants.count(0), "element_per_lane": variants.count(1),
+                              "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3)}}
 
 
 def main():
@@ -123,6 +155,8 @@ def main():
     ap.add_argument("--elements", default="", help="override the mesh, e.g. 256x64x128")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-model ensemble line")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -212,8 +246,11 @@ def main():
             adjoint_bench(lambda: P.maxwell_viscoelastic(elements=(64, 32, 64)), _abi.PARAM_LOG_VISCOSITY,
                           args.adjoint_steps, 50),
             adjoint_bench(lambda: P.mohr_coulomb_shear(elements=(64, 32, 64)), _abi.PARAM_COHESION,
-                          args.adjoint_steps, 50),
+                          args.adjoint_steps, 50, _abi.ADJ_CORNER_CLAMP),
         ]
+
+    if not args.no_ensemble and args.precision == "fp64":
+        out["ensemble"] = ensemble_bench(64, args.ensemble_steps)
 
     if not args.no_cpu and args.cpu_seconds > 0:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)

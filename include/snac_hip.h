@@ -231,16 +231,21 @@ typedef struct snh_kernel_times {
 int snh_profile(snh_context* ctx, int enable);
 
 /* Kernel-variant autotuner (no counterpart in the reference).  Times each
- * candidate implementation of the element pass (one tet per lane, or one
- * element per thread) on a scratch copy of the current state, a few launches
+ * candidate implementation of the element pass (one tet per lane, 6
+ * elements per wave; one element per thread keeping the tet kinematics in
+ * registers; the same recomputing them; one wave per tet index over 64
+ * elements) on a scratch copy of the current state, a few launches
  * each, and keeps the fastest for this context.  Call after
  * snh_timestep_zero; the state is not modified.  The choice can be forced
- * with the environment variable SNH_ELEM_VARIANT=0|1. */
-#define SNH_ELEM_TET_PER_LANE     0
-#define SNH_ELEM_ELEMENT_PER_LANE 1
+ * with the environment variable SNH_ELEM_VARIANT=0|1|2|3. */
+#define SNH_ELEM_TET_PER_LANE            0
+#define SNH_ELEM_ELEMENT_PER_LANE        1
+#define SNH_ELEM_ELEMENT_PER_LANE_LOWREG 2   /* kinematics recomputed, not kept */
+#define SNH_ELEM_TET_PER_WAVE            3   /* 64 elements per block, one wave per tet */
+#define SNH_ELEM_VARIANTS                4
 typedef struct snh_tune_report {
-    double   elem_ms[2];     /* average launch time per variant, ms (0 = not run) */
-    int32_t  elem_variant;   /* chosen */
+    double   elem_ms[SNH_ELEM_VARIANTS];  /* best launch time per variant, ms */
+    int32_t  elem_variant;                /* chosen */
 } snh_tune_report;
 int snh_autotune(snh_context* ctx, snh_tune_report* out);
 /* Force one element-pass variant (SNH_ELEM_*); both give identical results. */
@@ -278,9 +283,20 @@ int snh_set_schedule(snh_context* ctx, int32_t nsteps, const double* dt, const d
 #define SNH_PARAM_LOG_VISCOSITY 0   /* theta_e = ln(refvisc_e); needs SNH_FIELD_REFVISC */
 #define SNH_PARAM_COHESION      1   /* theta_e = cohesion offset of element e (Pa)      */
 
+/* Mohr-Coulomb corner regularisation.  The reference's single-surface return
+ * (plastic_BI/Constitutive.c:159-223) can move two principal stresses past
+ * each other; there the returned stress jumps with the eigenvectors of the
+ * trial stress and its exact derivative carries the divided difference
+ * (g_k - g_l) / (s_k - s_l) < 0, unbounded as the gap closes, which makes
+ * long reverse sweeps through plastic flow grow without bound.  With this
+ * flag that divided difference is clamped to [0, 1], the value of a return
+ * that stops at the corner.  Off: the exact derivative. */
+#define SNH_ADJ_CORNER_CLAMP 0x1
+
 typedef struct snh_adjoint_config {
     int32_t param;                /* SNH_PARAM_*                                  */
     int32_t checkpoint_interval;  /* steps per checkpoint segment, >= 1           */
+    int32_t flags;                /* SNH_ADJ_*                                    */
 } snh_adjoint_config;
 
 typedef struct snh_adjoint_times {
@@ -303,6 +319,23 @@ int snh_adjoint_reverse(snh_context* ctx, const double* coord_bar, const double*
 int snh_adjoint_get_times(snh_context* ctx, snh_adjoint_times* out);
 /* Release checkpoints, tape and adjoint buffers. */
 int snh_adjoint_end(snh_context* ctx);
+
+/* ---- ensembles -----------------------------------------------------------
+ * No counterpart in the reference.  Steps several contexts (independent
+ * models of any size, each with its own materials, BCs and element-pass
+ * variant, e.g. from snh_autotune) together: a HIP graph holds graph_steps
+ * steps of every member, each member's launches on its own stream as a
+ * parallel branch, so small meshes share the GPU.  One graph launch then
+ * advances every member graph_steps steps with no host work in between.
+ * Results are bit-identical to stepping each member with snh_step.
+ * graph_steps must be even (the coordinate buffers alternate every step);
+ * members must have no schedule and profiling off while in an ensemble. */
+typedef struct snh_ensemble snh_ensemble;
+int snh_ensemble_create(snh_context* const* members, int32_t n, int32_t graph_steps, snh_ensemble** out);
+/* nsteps steps of every member: whole graph launches, the rest with snh_step */
+int snh_ensemble_step(snh_ensemble* ens, int32_t nsteps);
+int snh_ensemble_synchronize(snh_ensemble* ens);
+int snh_ensemble_destroy(snh_ensemble* ens);
 
 /* Device-to-device copy bandwidth of the current GPU (GB/s, read + write
  * bytes), from a grid-stride copy of nbytes timed with HIP events over iters

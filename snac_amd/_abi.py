@@ -122,18 +122,24 @@ class KernelTimes(C.Structure):
 
 ELEM_TET_PER_LANE = 0
 ELEM_ELEMENT_PER_LANE = 1
+ELEM_ELEMENT_PER_LANE_LOWREG = 2
+ELEM_TET_PER_WAVE = 3
+ELEM_VARIANTS = 4
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("elem_ms", C.c_double * 2), ("elem_variant", C.c_int32)]
+    _fields_ = [("elem_ms", C.c_double * ELEM_VARIANTS), ("elem_variant", C.c_int32)]
 
 
 PARAM_LOG_VISCOSITY = 0
 PARAM_COHESION = 1
 
 
+ADJ_CORNER_CLAMP = 0x1
+
+
 class AdjointConfig(C.Structure):
-    _fields_ = [("param", C.c_int32), ("checkpoint_interval", C.c_int32)]
+    _fields_ = [("param", C.c_int32), ("checkpoint_interval", C.c_int32), ("flags", C.c_int32)]
 
 
 class AdjointTimes(C.Structure):
@@ -169,6 +175,10 @@ SIGNATURES = {
     "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
     "snh_set_schedule": (_I, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_int8)]),
+    "snh_ensemble_create": (_I, [C.POINTER(_P), C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "snh_ensemble_step": (_I, [_P, C.c_int32]),
+    "snh_ensemble_synchronize": (_I, [_P]),
+    "snh_ensemble_destroy": (_I, [_P]),
     "snh_adjoint_begin": (_I, [_P, C.POINTER(AdjointConfig)]),
     "snh_adjoint_forward": (_I, [_P, C.c_int32]),
     "snh_adjoint_reverse": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),

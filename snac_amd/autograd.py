@@ -51,12 +51,12 @@ def top_nodes(problem: Problem) -> np.ndarray:
 
 class SurfaceUplift(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, problem, param, nsteps, checkpoint_interval):
+    def forward(ctx, theta, problem, param, nsteps, checkpoint_interval, flags):
         p = with_parameter(problem, param, theta.detach().cpu().numpy())
         c = Context(p).Construct().Build().Initialise()
         top = top_nodes(p)
         y0 = c.state(fields=("coord",))["coord"][top, 1]
-        c.adjoint_begin(param, checkpoint_interval)
+        c.adjoint_begin(param, checkpoint_interval, flags)
         c.adjoint_forward(nsteps)
         y = c.state(fields=("coord",))["coord"][top, 1]
         ctx.snac = c
@@ -74,9 +74,9 @@ class SurfaceUplift(torch.autograd.Function):
             c.adjoint_end()
             c.Destroy()
             ctx.snac = None
-        return torch.as_tensor(g, dtype=grad_uplift.dtype, device=grad_uplift.device), None, None, None, None
+        return torch.as_tensor(g, dtype=grad_uplift.dtype, device=grad_uplift.device), None, None, None, None, None
 
 
 def surface_uplift(theta: torch.Tensor, problem: Problem, param: int, nsteps: int,
-                   checkpoint_interval: int = 50) -> torch.Tensor:
-    return SurfaceUplift.apply(theta, problem, param, nsteps, checkpoint_interval)
+                   checkpoint_interval: int = 50, flags: int = 0) -> torch.Tensor:
+    return SurfaceUplift.apply(theta, problem, param, nsteps, checkpoint_interval, flags)

@@ -110,6 +110,7 @@ class Context:
         r = _abi.TuneReport()
         self._check(self.lib.snh_autotune(self.h, C.byref(r)))
         return {"elem_ms_tet_per_lane": r.elem_ms[0], "elem_ms_element_per_lane": r.elem_ms[1],
+                "elem_ms_element_per_lane_lowreg": r.elem_ms[2], "elem_ms_tet_per_wave": r.elem_ms[3],
                 "elem_variant": r.elem_variant}
 
     def set_elem_variant(self, variant: int):
@@ -131,10 +132,11 @@ class Context:
         return self
 
     # -- adjoint ---------------------------------------------------------------
-    def adjoint_begin(self, param: int, checkpoint_interval: int = 50):
+    def adjoint_begin(self, param: int, checkpoint_interval: int = 50, flags: int = 0):
         """Start recording for a reverse sweep with respect to a per-element
-        field (_abi.PARAM_LOG_VISCOSITY or _abi.PARAM_COHESION)."""
-        cfg = _abi.AdjointConfig(int(param), int(checkpoint_interval))
+        field (_abi.PARAM_LOG_VISCOSITY or _abi.PARAM_COHESION); flags:
+        _abi.ADJ_CORNER_CLAMP."""
+        cfg = _abi.AdjointConfig(int(param), int(checkpoint_interval), int(flags))
         self._check(self.lib.snh_adjoint_begin(self.h, C.byref(cfg)))
         return self
 

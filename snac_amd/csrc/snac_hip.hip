@@ -35,8 +35,6 @@
 
 using namespace snh;
 
-__constant__ DevMaterial c_mat[SNH_MAX_MATERIALS];
-__constant__ DevBC c_bc[256];
 
 #define ELEM_BLOCK 128
 #define NODE_BLOCK 128
@@ -46,6 +44,16 @@ __constant__ DevBC c_bc[256];
 /* --------------------------------------------------------------------------
  * helpers
  * ------------------------------------------------------------------------ */
+/* material of element e: with one material (no per-element ids) the table
+   entry is uniform across the wave and its fields are scalar loads */
+template <bool MM>
+__device__ __forceinline__ const DevMaterial& mat_of(const StepArgs& a, int e) {
+    if constexpr (MM)
+        return a.mats[a.mat[e]];
+    else
+        return a.mats[0];
+}
+
 __device__ __forceinline__ int node_index(const StepArgs& a, int i, int j, int k) {
     return i + a.I * (j + a.J * k);
 }
@@ -325,9 +333,9 @@ __device__ __forceinline__ void soften(const DevMaterial& m, double ps, double& 
     for (uint32_t i = 0; i < m.nseg; i++) {
         const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
         if (ps >= pl1 && ps <= pl2) {
-            const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
-            const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
-            const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+            const double tgf = m.tgf[i];
+            const double tgd = m.tgd[i];
+            const double tgc = m.tgc[i];
             phi = m.fa[i] + tgf * (ps - pl1);
             psi = m.da[i] + tgd * (ps - pl1);
             coh = m.coh[i] + tgc * (ps - pl1);
@@ -412,18 +420,39 @@ __device__ __forceinline__ double tan_phi(const DevMaterial& m, double phi) {
    below 1 Pa.  For such a tet mc_return changes neither the stress nor the
    plastic strain, so its eigen-decomposition is skipped: the results are
    bit-identical. */
-__device__ __forceinline__ bool mc_surely_elastic(const double st[6], double sphi, double cohesion, double tc) {
+__device__ __forceinline__ bool mc_surely_elastic(const DevMaterial& m, const double st[6], double sphi, double cohesion,
+                                                  double tc) {
     const double p = (st[0] + st[1] + st[2]) * (1.0 / 3.0);
     const double d0 = st[0] - p, d1 = st[1] - p, d2 = st[2] - p;
     const double r2 = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (st[3] * st[3] + st[4] * st[4] + st[5] * st[5]);
     const double r = sqrt(r2 * (2.0 / 3.0));
-    const double anphi = (1.0 + sphi) / (1.0 - sphi);
+    const double anphi = sphi == m.sphi ? m.anphi0 : (1.0 + sphi) / (1.0 - sphi);
     const double smin = p - r, smax = p + r;
     const double scale = fabs(p) + r + fabs(cohesion) + fabs(tc);
     const double margin = 64.0 + 1e-9 * scale * (1.0 + anphi);
-    const double fs_lo = smin - smax * anphi + 2.0 * cohesion * sqrt(anphi);
+    const double fs_lo = smin - smax * anphi + 2.0 * cohesion * (sphi == m.sphi ? m.sqanphi0 : sqrt(anphi));
     const double ft_hi = smax - tc;
     return fs_lo > margin && ft_hi < -margin;
+}
+
+/* eigen-decomposition of the symmetric trial stress and Mohr-Coulomb return
+   for a tet that may yield; out of line, so that the registers it needs are
+   not reserved across the whole element loop (most tets never call it).
+   Returns whether the tet yielded; st then holds the returned stress. */
+__device__ __noinline__ bool mc_plastic_update(const DevMaterial& m, double* st, double sphi, double spsi,
+                                               double coh, double tc, double* dps) {
+    double S[3][3];
+    S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
+    S[0][1] = S[1][0] = st[3];
+    S[0][2] = S[2][0] = st[4];
+    S[1][2] = S[2][1] = st[5];
+    double d3[3], cn[3][3], dd = 0.0;
+    principal_stresses(S, d3, cn);
+    if (!mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dd)) return false;
+    st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
+    st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
+    *dps = dd;
+    return true;
 }
 
 /* viscosity of maxwell (:123-151) / viscoplastic_BI (:174-218); the
@@ -526,26 +555,32 @@ struct Soft {
     double phi, psi, coh, hard;
 };
 
-template <int RSET>
-__global__ void __launch_bounds__(64) k_elem(StepArgs a) {
-    __shared__ double sh_a[EPW][NTET];
-    __shared__ double sh_b[EPW][NTET];
-    __shared__ double sh_sp[EPW][NTET][5];   /* matched, phi, psi, coh, hard */
-    __shared__ double sh_ps[EPW][NTET];
+/* TW = false: one 64-lane block holds 6 elements x 10 tets.
+   TW = true ("tet per wave"): a 640-thread block holds 64 elements, wave w
+   taking tet w of all of them, so every stress and node load of a wave is
+   over 64 consecutive elements. */
+template <int RSET, bool TW, bool MM>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
+    constexpr int NE = TW ? 64 : EPW;       /* elements per block */
+    constexpr int NT = TW ? NTET + 1 : NTET; /* padded against LDS bank conflicts */
+    __shared__ double sh_a[NE][NT];
+    __shared__ double sh_b[NE][NT];
+    __shared__ double sh_sp[NE][NT][5];   /* matched, phi, psi, coh, hard */
+    __shared__ double sh_ps[NE][NT];
     const int lane = threadIdx.x;
-    /* lanes 60..63 carry no tet: they mirror tet 0 of element slot 0 but
-       never write LDS or memory */
-    const bool in_wave = lane < EPW * NTET;
-    const int le = in_wave ? lane / NTET : 0;
-    const int t = in_wave ? lane - le * NTET : 0;
-    const int e = xcd_block() * EPW + le;
+    /* (tet-per-lane) lanes 60..63 carry no tet: they mirror tet 0 of element
+       slot 0 but never write LDS or memory */
+    const bool in_wave = TW || lane < EPW * NTET;
+    const int le = TW ? (lane & 63) : (in_wave ? lane / NTET : 0);
+    const int t = TW ? (lane >> 6) : (in_wave ? lane - le * NTET : 0);
+    const int e = xcd_block() * NE + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
     const int ei = ec % a.EI, ej = (ec / a.EI) % a.EJ, ek = ec / (a.EI * a.EJ);
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
-    const DevMaterial& m = c_mat[a.mat ? a.mat[ec] : 0];
+    const DevMaterial& m = mat_of<MM>(a, ec);
 
     /* kinematics of this lane's tet (UpdateElement.c:84-219, StrainRate.c:65-184) */
     int nd[4];
@@ -602,9 +637,9 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
             for (uint32_t i = 0; i < m.nseg; i++) {
                 const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
                 if (ps >= pl1 && ps <= pl2) {
-                    const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
-                    const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
-                    const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+                    const double tgf = m.tgf[i];
+                    const double tgd = m.tgd[i];
+                    const double tgc = m.tgc[i];
                     own.phi = m.fa[i] + tgf * (ps - pl1);
                     own.psi = m.da[i] + tgd * (ps - pl1);
                     own.coh = m.coh[i] + tgc * (ps - pl1);
@@ -622,7 +657,11 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
             sh_ps[le][t] = ps;
         }
         __syncthreads();
-        if (is_pl || is_vp) {
+        if ((is_pl || is_vp) && matched != 0.0 && own.phi >= 0.0) {
+            /* the scan below ends with this tet's own parameters whenever its
+               plastic strain lies in a segment and the angle is not negative */
+            sp = own;
+        } else if (is_pl || is_vp) {
             sp.phi = is_pl ? 30.0 : 0.0;   /* plastic_BI :78-80 defaults */
             sp.coh = is_pl ? 30.0e6 : 0.0;
             for (int q = 0; q <= t; q++) {
@@ -661,16 +700,13 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
         if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
         double sphi, spsi, dps = 0.0;
         angle_terms(m, sp.phi, sp.psi, sphi, spsi);
-        if (!mc_surely_elastic(st, sphi, coh, tc)) {
-            double d3[3], cn[3][3];
-            principal_stresses(S, d3, cn);
-            if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
-                ps = ps + dps;
-                st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-                st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
-            }
+        bool changed = false;
+        if (!mc_surely_elastic(m, st, sphi, coh, tc) && mc_plastic_update(m, st, sphi, spsi, coh, tc, &dps)) {
+            ps = ps + dps;
+            changed = true;
         }
-        if (active) a.ps_out[(size_t)t * E + ec] = ps;
+        /* in place, an unchanged value need not be stored */
+        if (active && (changed || a.ps_out != a.ps || ps != sh_ps[le][t])) a.ps_out[(size_t)t * E + ec] = ps;
     } else if (is_vp) {
         double trace_strain = sn[0] + sn[1] + sn[2];
         double d0, d1, d2;
@@ -747,22 +783,25 @@ __global__ void __launch_bounds__(64) k_elem(StepArgs a) {
  * iterative work, e.g. the plastic eigen-solve, one element per lane
  * diverges less than one tet per lane).  Identical arithmetic.
  * ------------------------------------------------------------------------ */
-template <int RSET>
+template <int RSET, bool LOWREG, bool MM>
 __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
     const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
-    const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+    const DevMaterial& m = mat_of<MM>(a, e);
 
     double X[8][3], U[8][3];
     load_hex(a, a.xc, ei, ej, ek, X);
     load_hex(a, a.v, ei, ej, ek, U);
 
-    /* pass 1: kinematics of the 10 tets */
-    double sr[NTET][6], w[NTET][3], vol[NTET];
-    double evol = 0.0;
+    /* pass 1: kinematics of the 10 tets.  LOWREG keeps none of it and
+       recomputes it in pass 2 (bit-identical), trading FP64 work for
+       registers */
+    double sr[LOWREG ? 1 : NTET][6], w[LOWREG ? 1 : NTET][3], vol[LOWREG ? 1 : NTET];
+    double trace[LOWREG ? 1 : NTET];
+    double evol = 0.0, partial = 0.0;
 #pragma unroll
     for (int t = 0; t < NTET; t++) {
         double Xt[4][3], Ut[4][3];
@@ -775,26 +814,25 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             }
         TetKin k;
         tet_kinematics(Xt, Ut, k);
-        vol[t] = k.V;
         evol += k.V;
+        const double trc = snh_div3(k.sr[0] + k.sr[1] + k.sr[2]);
+        partial += trc * k.V;
+        if constexpr (!LOWREG) {
+            vol[t] = k.V;
+            trace[t] = trc;
 #pragma unroll
-        for (int c = 0; c < 6; c++) sr[t][c] = k.sr[c];
+            for (int c = 0; c < 6; c++) sr[t][c] = k.sr[c];
 #pragma unroll
-        for (int c = 0; c < 3; c++) w[t][c] = k.w[c];
+            for (int c = 0; c < 3; c++) w[t][c] = k.w[c];
+        }
     }
     evol *= 0.5;                        /* UpdateElement.c:267-268 */
     const double elemVolume = 2.0 * evol; /* StrainRate.c:57, Stress.c:68 */
-    double partial = 0.0;
-    double trace[NTET];
-#pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        trace[t] = snh_div3(sr[t][0] + sr[t][1] + sr[t][2]);
-        partial += trace[t] * vol[t];
-    }
     const double srcorr = partial / elemVolume;
 
     /* pass 2: rotation, strain, constitutive */
-    double sig[NTET][6];
+    double sig[LOWREG ? 1 : NTET][6];
+    double trs[LOWREG ? NTET : 1], pst = 0.0;
     const size_t E = (size_t)a.E;
     /* plastic_BI / viscoplastic_BI keep their softened parameters across the
        tets of an element */
@@ -804,15 +842,38 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
     const double cohoff = a.cohoff ? a.cohoff[e] : 0.0;
 #pragma unroll
     for (int t = 0; t < NTET; t++) {
+        TetKin kr;
+        const double *srt, *wt;
+        double volt, tracet;
+        if constexpr (LOWREG) {
+            double Xt[4][3], Ut[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    Xt[q][d] = X[kTetraToNode[t][q]][d];
+                    Ut[q][d] = U[kTetraToNode[t][q]][d];
+                }
+            tet_kinematics(Xt, Ut, kr);
+            srt = kr.sr;
+            wt = kr.w;
+            volt = kr.V;
+            tracet = snh_div3(kr.sr[0] + kr.sr[1] + kr.sr[2]);
+        } else {
+            srt = sr[t];
+            wt = w[t];
+            volt = vol[t];
+            tracet = trace[t];
+        }
         double st[6];
 #pragma unroll
         for (int c = 0; c < 6; c++) st[c] = a.stress[((size_t)c * NTET + t) * E + e];
-        rotate_stress(st, w[t], dt_rot);
+        rotate_stress(st, wt, dt_rot);
         double s6[6];
 #pragma unroll
-        for (int c = 0; c < 3; c++) s6[c] = sr[t][c] + (-1.0 * trace[t] + srcorr);
+        for (int c = 0; c < 3; c++) s6[c] = srt[c] + (-1.0 * tracet + srcorr);
 #pragma unroll
-        for (int c = 3; c < 6; c++) s6[c] = sr[t][c];
+        for (int c = 3; c < 6; c++) s6[c] = srt[c];
         double sn[6];
 #pragma unroll
         for (int c = 0; c < 6; c++) sn[c] = s6[c] * dt;
@@ -849,15 +910,8 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             double sphi, spsi;
             angle_terms(m, pphi, ppsi, sphi, spsi);
             double dps = 0.0;
-            if (!mc_surely_elastic(st, sphi, coh, ptc)) {
-                double d3[3], cn[3][3];
-                principal_stresses(S, d3, cn);
-                if (mc_return(m, S, d3, cn, sphi, spsi, coh, ptc, 0.0, false, dps)) {
-                    plasticStrain = plasticStrain + dps;
-                    st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-                    st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
-                }
-            }
+            if (!mc_surely_elastic(m, st, sphi, coh, ptc) && mc_plastic_update(m, st, sphi, spsi, coh, ptc, &dps))
+                plasticStrain = plasticStrain + dps;
             /* in place, an unchanged value need not be stored */
             if (a.ps_out != a.ps || plasticStrain != a.ps[(size_t)t * E + e])
                 a.ps_out[(size_t)t * E + e] = plasticStrain;
@@ -909,7 +963,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
                 Xp[q][2] = a.xp[2 * a.N + nd];
             }
             double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
-            double trace_strain = vol[t] / old_volume - 1.0;
+            double trace_strain = volt / old_volume - 1.0;
             double d0, d1, d2;
             devstrain(sn, trace_strain, d0, d1, d2);
             double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
@@ -927,25 +981,44 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             st[4] += sh2 * sn[4];
             st[5] += sh2 * sn[5];
         }
+        if constexpr (LOWREG) {
+            /* store now; pass 3 adds the averaged trace to the diagonal */
+            trs[t] = snh_div3(st[0] + st[1] + st[2]);
+            pst += trs[t] * volt;
 #pragma unroll
-        for (int c = 0; c < 6; c++) sig[t][c] = st[c];
+            for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + e] = st[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; c++) sig[t][c] = st[c];
+        }
     }
 
     /* pass 3: stress trace averaging (Stress.c:87-137) */
-    double pst = 0.0, tr[NTET];
+    if constexpr (LOWREG) {
+        const double avgTrace = pst / elemVolume;
 #pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        tr[t] = snh_div3(sig[t][0] + sig[t][1] + sig[t][2]);
-        pst += tr[t] * vol[t];
-    }
-    const double avgTrace = pst / elemVolume;
+        for (int t = 0; t < NTET; t++)
 #pragma unroll
-    for (int t = 0; t < NTET; t++) {
-        sig[t][0] += -1.0 * tr[t] + avgTrace;
-        sig[t][1] += -1.0 * tr[t] + avgTrace;
-        sig[t][2] += -1.0 * tr[t] + avgTrace;
+            for (int c = 0; c < 3; c++) {
+                double* p = &a.stress_out[((size_t)c * NTET + t) * E + e];
+                *p = *p + (-1.0 * trs[t] + avgTrace);
+            }
+    } else {
+        double tr[NTET];
 #pragma unroll
-        for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + e] = sig[t][c];
+        for (int t = 0; t < NTET; t++) {
+            tr[t] = snh_div3(sig[t][0] + sig[t][1] + sig[t][2]);
+            pst += tr[t] * vol[t];
+        }
+        const double avgTrace = pst / elemVolume;
+#pragma unroll
+        for (int t = 0; t < NTET; t++) {
+            sig[t][0] += -1.0 * tr[t] + avgTrace;
+            sig[t][1] += -1.0 * tr[t] + avgTrace;
+            sig[t][2] += -1.0 * tr[t] + avgTrace;
+#pragma unroll
+            for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + e] = sig[t][c];
+        }
     }
 }
 
@@ -1000,6 +1073,7 @@ __global__ void k_volumes(StepArgs a, double* __restrict__ out) {
  * ------------------------------------------------------------------------ */
 #define GETSIGN(val) ((val) < 0 ? -1.0 : (val) > 0 ? +1.0 : 0.0)
 
+template <bool MM>
 __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
     const int n = xcd_block() * blockDim.x + threadIdx.x;
@@ -1015,7 +1089,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
             const int ei = i - 1 + (slot & 1), ej = j - 1 + ((slot >> 1) & 1), ek = k - 1 + ((slot >> 2) & 1);
             if (ei < 0 || ej < 0 || ek < 0 || ei >= a.EI || ej >= a.EJ || ek >= a.EK) continue;
             const int e = ei + a.EI * (ej + a.EJ * ek);
-            const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+            const DevMaterial& m = mat_of<MM>(a, e);
             const double inertialDensity = m.lam2mu / (c * c);
             const double rho = m.rho_eff;
             double X[8][3];
@@ -1092,7 +1166,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
         v2 += dt * f2 / im;
         const uint32_t code = a.bc[n];
         if (code) {
-            const DevBC& b = c_bc[code];
+            const DevBC& b = a.bcs[code];
             if (b.mask & 1u) v0 = b.val[0];
             if (b.mask & 2u) v1 = b.val[1];
             if (b.mask & 4u) v2 = b.val[2];
@@ -1252,6 +1326,8 @@ struct snh_context {
     double* part_min;
     double* part_vmax;
     DevScalars* scal;
+    DevMaterial* dmat;   /* material table [SNH_MAX_MATERIALS] */
+    DevBC* dbc;          /* velocity-condition table [256], entry 0 unused */
     double* tmp;    /* download staging [6][10][E] */
     int n_geo_blocks, n_node_blocks;
     unsigned rset;  /* rheologies present */
@@ -1326,11 +1402,19 @@ static void fill_dev_material(const snh_material& s, DevMaterial& d) {
         d.da[i] = s.dilationAngle[i];
         d.coh[i] = s.cohesion[i];
     }
+    for (unsigned i = 0; i < s.nsegments && i < 8; i++) {
+        const double pl1 = s.plstrain[i], pl2 = s.plstrain[i + 1];
+        d.tgf[i] = (s.frictionAngle[i + 1] - s.frictionAngle[i]) / (pl2 - pl1);
+        d.tgd[i] = (s.dilationAngle[i + 1] - s.dilationAngle[i]) / (pl2 - pl1);
+        d.tgc[i] = (s.cohesion[i + 1] - s.cohesion[i]) / (pl2 - pl1);
+    }
     d.ten_off = s.ten_off;
     const double degrad = M_PI / 180.0;
     d.sphi = sin(s.frictionAngle[0] * degrad);
     d.spsi = sin(s.dilationAngle[0] * degrad);
     d.tanphi = tan(s.frictionAngle[0] * degrad);
+    d.anphi0 = (1.0 + d.sphi) / (1.0 - d.sphi);
+    d.sqanphi0 = sqrt(d.anphi0);
     d.vis_min = s.vis_min;
     d.vis_max = s.vis_max;
     d.refvisc = s.refvisc;
@@ -1378,6 +1462,8 @@ static StepArgs make_args(snh_context* c) {
     a.part_min = c->part_min;
     a.part_vmax = c->part_vmax;
     a.scal = c->scal;
+    a.mats = c->dmat;
+    a.bcs = c->dbc;
     a.I = c->I; a.J = c->J; a.K = c->K;
     a.EI = c->EI; a.EJ = c->EJ; a.EK = c->EK;
     a.N = c->N; a.E = c->E;
@@ -1390,7 +1476,8 @@ static StepArgs make_args(snh_context* c) {
 
 static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
-                    c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp};
+                    c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
+                    c->dmat, c->dbc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1483,6 +1570,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     ALLOC(c->part_min, (size_t)c->n_geo_blocks * sizeof(double));
     ALLOC(c->part_vmax, (size_t)c->n_node_blocks * sizeof(double));
     ALLOC(c->scal, sizeof(DevScalars));
+    ALLOC(c->dmat, SNH_MAX_MATERIALS * sizeof(DevMaterial));
+    ALLOC(c->dbc, 256 * sizeof(DevBC));
 #undef ALLOC
     if (!ok) {
         free_all(c);
@@ -1543,9 +1632,10 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
 
     int rc = SNH_OK;
     do {
-        if (hipMemcpyToSymbol(HIP_SYMBOL(c_mat), dm.data(), sizeof(DevMaterial) * SNH_MAX_MATERIALS) != hipSuccess ||
-            hipMemcpyToSymbol(HIP_SYMBOL(c_bc), table.data(), sizeof(DevBC) * table.size()) != hipSuccess) {
-            rc = set_err(SNH_EHIP, "constant upload failed");
+        if (hipMemcpy(c->dmat, dm.data(), sizeof(DevMaterial) * SNH_MAX_MATERIALS, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(c->dbc, table.data(), sizeof(DevBC) * table.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = set_err(SNH_EHIP, "material / BC table upload failed");
             break;
         }
         if (hipMemcpy(c->x[0], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1582,7 +1672,10 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     c->sched_damp = nullptr;
     c->sched_base = 0;
     c->sched_n = 0;
-    if (const char* ev = getenv("SNH_ELEM_VARIANT")) c->elem_variant = atoi(ev) == 1 ? 1 : 0;
+    if (const char* ev = getenv("SNH_ELEM_VARIANT")) {
+        const int v = atoi(ev);
+        c->elem_variant = (v >= 0 && v < SNH_ELEM_VARIANTS) ? v : 0;
+    }
     c->zero_done = false;
     c->rotation_pending = false;
     c->timeStep = 0;
@@ -1704,6 +1797,7 @@ static int launch_elem(snh_context* c) {
 }
 static int launch_elem_args(snh_context* c, StepArgs a) {
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
+    const bool mm = a.mat != nullptr;
     if (c->cfg.precision == SNH_PRECISION_MIXED) {
         switch (c->rset) {
         case RS_ELASTIC: hipLaunchKernelGGL(k_elem_mx<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
@@ -1716,31 +1810,53 @@ static int launch_elem_args(snh_context* c, StepArgs a) {
         HIPCHK(hipGetLastError());
         return SNH_OK;
     }
-    if (c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE) {
+    if (c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE || c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG) {
         dim3 g2((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), b2(ELEM_BLOCK);
+        const bool lr = c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG;
+#define SEQ_LAUNCH(R)                                                                         \
+    do {                                                                                      \
+        if (mm) {                                                                             \
+            if (lr) hipLaunchKernelGGL((k_elem_seq<R, true, true>), g2, b2, 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_elem_seq<R, false, true>), g2, b2, 0, c->stream, a);   \
+        } else {                                                                              \
+            if (lr) hipLaunchKernelGGL((k_elem_seq<R, true, false>), g2, b2, 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_elem_seq<R, false, false>), g2, b2, 0, c->stream, a);  \
+        }                                                                                     \
+    } while (0)
         switch (c->rset) {
-        case RS_ELASTIC: hipLaunchKernelGGL(k_elem_seq<RS_ELASTIC>, g2, b2, 0, c->stream, a); break;
-        case RS_PLASTIC: hipLaunchKernelGGL(k_elem_seq<RS_PLASTIC>, g2, b2, 0, c->stream, a); break;
-        case RS_MAXWELL: hipLaunchKernelGGL(k_elem_seq<RS_MAXWELL>, g2, b2, 0, c->stream, a); break;
-        case RS_VISCOPLASTIC: hipLaunchKernelGGL(k_elem_seq<RS_VISCOPLASTIC>, g2, b2, 0, c->stream, a); break;
-        default:
-            hipLaunchKernelGGL(k_elem_seq<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL | RS_VISCOPLASTIC>, g2, b2, 0,
-                               c->stream, a);
-            break;
+        case RS_ELASTIC: SEQ_LAUNCH(RS_ELASTIC); break;
+        case RS_PLASTIC: SEQ_LAUNCH(RS_PLASTIC); break;
+        case RS_MAXWELL: SEQ_LAUNCH(RS_MAXWELL); break;
+        case RS_VISCOPLASTIC: SEQ_LAUNCH(RS_VISCOPLASTIC); break;
+        default: SEQ_LAUNCH(RS_ELASTIC | RS_PLASTIC | RS_MAXWELL | RS_VISCOPLASTIC); break;
         }
+#undef SEQ_LAUNCH
         HIPCHK(hipGetLastError());
         return SNH_OK;
     }
-    switch (c->rset) {
-    case RS_ELASTIC: hipLaunchKernelGGL(k_elem<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
-    case RS_PLASTIC: hipLaunchKernelGGL(k_elem<RS_PLASTIC>, grid, block, 0, c->stream, a); break;
-    case RS_MAXWELL: hipLaunchKernelGGL(k_elem<RS_MAXWELL>, grid, block, 0, c->stream, a); break;
-    case RS_VISCOPLASTIC: hipLaunchKernelGGL(k_elem<RS_VISCOPLASTIC>, grid, block, 0, c->stream, a); break;
-    default:
-        hipLaunchKernelGGL(k_elem<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL | RS_VISCOPLASTIC>, grid, block, 0, c->stream,
-                           a);
-        break;
+    const bool tw = c->elem_variant == SNH_ELEM_TET_PER_WAVE;
+    if (tw) {
+        grid = dim3((c->E + 63) / 64);
+        block = dim3(640);
     }
+#define TL_LAUNCH(R)                                                                          \
+    do {                                                                                      \
+        if (mm) {                                                                             \
+            if (tw) hipLaunchKernelGGL((k_elem<R, true, true>), grid, block, 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_elem<R, false, true>), grid, block, 0, c->stream, a);  \
+        } else {                                                                              \
+            if (tw) hipLaunchKernelGGL((k_elem<R, true, false>), grid, block, 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_elem<R, false, false>), grid, block, 0, c->stream, a); \
+        }                                                                                     \
+    } while (0)
+    switch (c->rset) {
+    case RS_ELASTIC: TL_LAUNCH(RS_ELASTIC); break;
+    case RS_PLASTIC: TL_LAUNCH(RS_PLASTIC); break;
+    case RS_MAXWELL: TL_LAUNCH(RS_MAXWELL); break;
+    case RS_VISCOPLASTIC: TL_LAUNCH(RS_VISCOPLASTIC); break;
+    default: TL_LAUNCH(RS_ELASTIC | RS_PLASTIC | RS_MAXWELL | RS_VISCOPLASTIC); break;
+    }
+#undef TL_LAUNCH
     HIPCHK(hipGetLastError());
     return SNH_OK;
 }
@@ -1758,8 +1874,10 @@ static int launch_node(snh_context* c) {
     if (si >= 0 && c->sched_damp) a.damp_sched = c->sched_damp + (size_t)si * c->N;
     if (c->cfg.precision == SNH_PRECISION_MIXED)
         hipLaunchKernelGGL(k_node_mx, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    else if (a.mat)
+        hipLaunchKernelGGL(k_node<true>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     else
-        hipLaunchKernelGGL(k_node, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_node<false>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     c->cur ^= 1;
     return SNH_OK;
@@ -2097,7 +2215,7 @@ extern "C" int snh_measure_copy_bandwidth(size_t nbytes, int iters, double* gbps
 
 extern "C" int snh_set_elem_variant(snh_context* c, int32_t v) {
     if (!c) return set_err(SNH_EINVAL, "null context");
-    if (v != SNH_ELEM_TET_PER_LANE && v != SNH_ELEM_ELEMENT_PER_LANE) return set_err(SNH_EINVAL, "unknown element variant");
+    if (v < 0 || v >= SNH_ELEM_VARIANTS) return set_err(SNH_EINVAL, "unknown element variant");
     c->elem_variant = v;
     return SNH_OK;
 }
@@ -2125,7 +2243,7 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
     const int saved = c->elem_variant;
     const int reps = 3;
     int rc = SNH_OK;
-    for (int v = 0; v < 2 && rc == SNH_OK; v++) {
+    for (int v = 0; v < SNH_ELEM_VARIANTS && rc == SNH_OK; v++) {
         c->elem_variant = v;
         StepArgs a = make_args(c);
         a.stress = st;
@@ -2160,7 +2278,10 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
         c->elem_variant = saved;
         return rc;
     }
-    c->elem_variant = rep.elem_ms[1] < rep.elem_ms[0] ? SNH_ELEM_ELEMENT_PER_LANE : SNH_ELEM_TET_PER_LANE;
+    int best = 0;
+    for (int v = 1; v < SNH_ELEM_VARIANTS; v++)
+        if (rep.elem_ms[v] < rep.elem_ms[best]) best = v;
+    c->elem_variant = best;
     if (getenv("SNH_ELEM_VARIANT")) c->elem_variant = saved;
     rep.elem_variant = c->elem_variant;
     if (out) *out = rep;
@@ -2168,3 +2289,168 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
 }
 
 #include "snh_adjoint_host.h"
+
+/* ==========================================================================
+ * ensembles: one HIP graph steps every member, members on parallel branches
+ * ======================================================================== */
+struct snh_ensemble {
+    std::vector<snh_context*> m;
+    std::vector<int> cur0;      /* members' coordinate-buffer index at capture */
+    int graph_steps;
+    hipStream_t origin;
+    hipEvent_t fork;
+    std::vector<hipEvent_t> join;
+    /* [0]: buffers as at capture, [1]: every member's buffers swapped */
+    hipGraph_t graph[2];
+    hipGraphExec_t exec[2];
+};
+
+static void ens_free(snh_ensemble* e) {
+    if (!e) return;
+    for (int p = 0; p < 2; p++) {
+        if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
+        if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+    }
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    for (hipEvent_t j : e->join)
+        if (j) (void)hipEventDestroy(j);
+    if (e->origin) (void)hipStreamDestroy(e->origin);
+    delete e;
+}
+
+extern "C" int snh_ensemble_create(snh_context* const* members, int32_t n, int32_t graph_steps,
+                                   snh_ensemble** out) {
+    if (!members || !out || n < 1) return set_err(SNH_EINVAL, "bad ensemble arguments");
+    if (graph_steps < 2 || (graph_steps & 1)) return set_err(SNH_EINVAL, "graph_steps must be even and >= 2");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        snh_context* c = members[i];
+        if (!c) return set_err(SNH_EINVAL, "null ensemble member %d", i);
+        if (!c->zero_done) return set_err(SNH_ESTATE, "ensemble member %d before snh_timestep_zero", i);
+        if (c->sched_n > 0 || c->profiling || c->adj)
+            return set_err(SNH_ESTATE, "ensemble member %d has a schedule, profiling or an adjoint", i);
+        for (int j = 0; j < i; j++)
+            if (members[j] == c || members[j]->stream == c->stream)
+                return set_err(SNH_EINVAL, "ensemble members need distinct contexts and streams");
+    }
+    snh_ensemble* e = new (std::nothrow) snh_ensemble();
+    if (!e) return set_err(SNH_ENOMEM, "out of host memory");
+    e->m.assign(members, members + n);
+    e->graph_steps = graph_steps;
+    e->origin = nullptr;
+    e->fork = nullptr;
+    e->graph[0] = e->graph[1] = nullptr;
+    e->exec[0] = e->exec[1] = nullptr;
+    e->join.assign(n, nullptr);
+    e->cur0.resize(n);
+    for (int i = 0; i < n; i++) e->cur0[i] = members[i]->cur;
+    int rc = SNH_OK;
+    do {
+        if (hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) {
+            rc = set_err(SNH_EHIP, "ensemble stream/event creation failed");
+            break;
+        }
+        for (int i = 0; i < n && rc == SNH_OK; i++)
+            if (hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess)
+                rc = set_err(SNH_EHIP, "ensemble event creation failed");
+        if (rc) break;
+        /* members' pending work first: the graph then starts from a settled state */
+        for (snh_context* c : e->m)
+            if (hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(SNH_EHIP, "member sync failed");
+        if (rc) break;
+        /* capture records the launches without running them; the host-side
+           buffer index is set for each of the two parities and restored */
+        for (int par = 0; par < 2 && rc == SNH_OK; par++) {
+            for (int i = 0; i < n; i++) e->m[i]->cur = e->cur0[i] ^ par;
+            if (hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = set_err(SNH_EHIP, "hipStreamBeginCapture failed");
+                break;
+            }
+            int crc = SNH_OK;
+            if (hipEventRecord(e->fork, e->origin) != hipSuccess) crc = SNH_EHIP;
+            std::vector<uint32_t> t0(n);
+            for (int i = 0; i < n && crc == SNH_OK; i++) {
+                snh_context* c = e->m[i];
+                t0[i] = c->timeStep;
+                if (hipStreamWaitEvent(c->stream, e->fork, 0) != hipSuccess) crc = SNH_EHIP;
+                if (crc == SNH_OK) crc = snh_step(c, graph_steps);
+                if (crc == SNH_OK && hipEventRecord(e->join[i], c->stream) != hipSuccess) crc = SNH_EHIP;
+                if (crc == SNH_OK && hipStreamWaitEvent(e->origin, e->join[i], 0) != hipSuccess) crc = SNH_EHIP;
+            }
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(e->origin, &g);
+            for (int i = 0; i < n; i++) e->m[i]->timeStep = t0[i];
+            if (crc != SNH_OK || ce != hipSuccess || !g) {
+                if (g) (void)hipGraphDestroy(g);
+                rc = set_err(SNH_EHIP, "ensemble capture failed");
+                break;
+            }
+            e->graph[par] = g;
+            if (hipGraphInstantiate(&e->exec[par], g, nullptr, nullptr, 0) != hipSuccess)
+                rc = set_err(SNH_EHIP, "hipGraphInstantiate failed");
+        }
+        for (int i = 0; i < n; i++) e->m[i]->cur = e->cur0[i];
+    } while (0);
+    if (rc) {
+        ens_free(e);
+        return rc;
+    }
+    *out = e;
+    return SNH_OK;
+}
+
+extern "C" int snh_ensemble_step(snh_ensemble* e, int32_t nsteps) {
+    if (!e) return set_err(SNH_EINVAL, "null ensemble");
+    if (nsteps < 0) return set_err(SNH_EINVAL, "negative step count");
+    /* every member's buffers as at capture, or every member's swapped:
+       otherwise (members stepped on their own meanwhile) plain steps */
+    const int par = e->m[0]->cur ^ e->cur0[0];
+    bool same = true;
+    for (size_t i = 0; i < e->m.size(); i++) same = same && ((e->m[i]->cur ^ e->cur0[i]) == par);
+    const int reps = same ? nsteps / e->graph_steps : 0;
+    const int rest = nsteps - reps * e->graph_steps;
+    if (reps > 0)
+        for (size_t i = 0; i < e->m.size(); i++) {   /* graph after the members' earlier work */
+            HIPCHK(hipEventRecord(e->join[i], e->m[i]->stream));
+            HIPCHK(hipStreamWaitEvent(e->origin, e->join[i], 0));
+        }
+    for (int r = 0; r < reps; r++) {
+        HIPCHK(hipGraphLaunch(e->exec[par], e->origin));
+        for (snh_context* c : e->m) {
+            c->timeStep += e->graph_steps;
+            c->rotation_pending = true;
+        }
+    }
+    if (rest) {
+        /* the members' own streams wait for the graph, then run the rest */
+        HIPCHK(hipEventRecord(e->fork, e->origin));
+        for (size_t i = 0; i < e->m.size(); i++) {
+            snh_context* c = e->m[i];
+            HIPCHK(hipStreamWaitEvent(c->stream, e->fork, 0));
+            int rc = snh_step(c, rest);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(e->join[i], c->stream));
+            HIPCHK(hipStreamWaitEvent(e->origin, e->join[i], 0));
+        }
+    } else {
+        /* later work on a member's own stream follows the graph */
+        HIPCHK(hipEventRecord(e->fork, e->origin));
+        for (snh_context* c : e->m) HIPCHK(hipStreamWaitEvent(c->stream, e->fork, 0));
+    }
+    return SNH_OK;
+}
+
+extern "C" int snh_ensemble_synchronize(snh_ensemble* e) {
+    if (!e) return set_err(SNH_EINVAL, "null ensemble");
+    HIPCHK(hipStreamSynchronize(e->origin));
+    for (snh_context* c : e->m) HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+extern "C" int snh_ensemble_destroy(snh_ensemble* e) {
+    if (!e) return SNH_OK;
+    (void)hipStreamSynchronize(e->origin);
+    ens_free(e);
+    return SNH_OK;
+}

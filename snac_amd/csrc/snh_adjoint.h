@@ -68,6 +68,7 @@ struct AdjArgs {
     double* grad;         /* [E]    accumulated parameter gradient             */
     double* nb;           /* [E][8][NBC] per element-node contributions        */
     int param;            /* SNH_PARAM_*                                       */
+    int flags;            /* SNH_ADJ_*                                         */
 };
 
 struct NodeAdjArgs {
@@ -80,6 +81,7 @@ struct NodeAdjArgs {
     const double* tape_m;
     const uint8_t* tape_code;
     const uint8_t* bc;
+    const DevBC* bcs;        /* velocity-condition table */
     const DevScalars* scal;  /* scalars of step n */
     double demf;
     int I, J, K, EI, EJ, EK, N, E;
@@ -253,7 +255,7 @@ SNH_HD MCBranch mc_branch(const DevMaterial& m, const double s[3], double sphi, 
    6 components); psb: d/d(plastic strain after the step).  Writes d/d(trial
    stress) to Sb and accumulates d/d(cohesion) and d/d(tension cut-off). */
 SNH_HD void mc_return_rev(const DevMaterial& m, const MCBranch& br, const double s[3], const double cn[3][3],
-                          const double ob[6], double psb, double Sb[6], double& cohb, double& tcb) {
+                          const double ob[6], double psb, double Sb[6], double& cohb, double& tcb, bool clamp) {
     const double a1 = m.lambda + 2.0 * m.mu;
     const double a2 = m.lambda;
     double beta[3], K;
@@ -306,6 +308,7 @@ SNH_HD void mc_return_rev(const DevMaterial& m, const MCBranch& br, const double
             const double gap = s[kk] - s[l];
             double G = 1.0;
             if (fabs(gap) > 1e-14 * (fabs(s[kk]) + fabs(s[l]))) G = 1.0 - br.alam * (beta[kk] - beta[l]) / gap;
+            if (clamp) G = G < 0.0 ? 0.0 : (G > 1.0 ? 1.0 : G);
             Db[kk][l] = Db[l][kk] = G * Mb[kk][l];
         }
     /* Sb = C Db C^T */
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_adj(NodeAdjArgs A) {
     }
     const uint32_t code = A.bc[n];
     if (code) {
-        const DevBC& b = c_bc[code];
+        const DevBC& b = A.bcs[code];
         if (b.mask & 1u) vp[0] = 0.0;
         if (b.mask & 2u) vp[1] = 0.0;
         if (b.mask & 4u) vp[2] = 0.0;
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
     const double csound = a.scal->speedOfSound;
-    const DevMaterial& m = c_mat[a.mat ? a.mat[ec] : 0];
+    const DevMaterial& m = a.mats[a.mat ? a.mat[ec] : 0];
 
     /* ---------------- forward recomputation (as k_elem) ---------------- */
     int nd[4];
@@ -471,9 +474,9 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
             for (uint32_t i = 0; i < m.nseg; i++) {
                 const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
                 if (ps >= pl1 && ps <= pl2) {
-                    const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
-                    const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
-                    const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+                    const double tgf = m.tgf[i];
+                    const double tgd = m.tgd[i];
+                    const double tgc = m.tgc[i];
                     own_phi = m.fa[i] + tgf * (ps - pl1);
                     own_psi = m.da[i] + tgd * (ps - pl1);
                     own_coh = m.coh[i] + tgc * (ps - pl1);
@@ -516,7 +519,7 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
             S[0][1] = S[1][0] = st[3];
             S[0][2] = S[2][0] = st[4];
             S[1][2] = S[2][1] = st[5];
-            if (!mc_surely_elastic(st, sphi, coh, tc)) {
+            if (!mc_surely_elastic(m, st, sphi, coh, tc)) {
                 principal_stresses(S, d3, cn);
                 br = mc_branch(m, d3, sphi, spsi, coh, tc);
                 double dps = 0.0;
@@ -633,7 +636,7 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
         double Sb[6];
         if (br.branch) {
             double tcb = 0.0;
-            mc_return_rev(m, br, d3, cn, sg, psb, Sb, cohb, tcb);
+            mc_return_rev(m, br, d3, cn, sg, psb, Sb, cohb, tcb, (A.flags & SNH_ADJ_CORNER_CLAMP) != 0);
             if (m.ten_off == 0.0 && phi > 0.0) cohb += tcb / tan_phi(m, phi);
         } else {
 #pragma unroll

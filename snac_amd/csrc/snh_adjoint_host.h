@@ -41,6 +41,8 @@ struct AdjointCtx {
     double *sbar = nullptr, *psbar = nullptr, *xbar = nullptr, *vbar = nullptr;
     double *fbar = nullptr, *mbar = nullptr, *nb = nullptr, *grad = nullptr;
     double forward_ms = 0.0, replay_ms = 0.0, reverse_ms = 0.0;
+    int tape_seg = -1;          /* segment whose steps the tape holds      */
+    int tape_len = 0;           /* steps of it recorded                    */
 };
 
 static void adj_free(AdjointCtx* A) {
@@ -132,10 +134,73 @@ extern "C" int snh_adjoint_begin(snh_context* c, const snh_adjoint_config* cfg) 
     return SNH_OK;
 }
 
-static int adj_save_checkpoint(snh_context* c, AdjointCtx* A) {
+/* run tape steps i0 .. i1-1 of the segment entered at timeStep t0: the
+   forward kernels read slot i and write slot i+1; k_node also writes the
+   undamped force, mass and damping branch; hist[i] keeps the scalars */
+static int adj_run_tape(snh_context* c, AdjointCtx* A, uint32_t t0, int i0, int i1) {
+    const size_t E = c->E, N = c->N;
+    int rc = SNH_OK;
+    for (int i = i0; i < i1 && rc == SNH_OK; i++) {
+        const int si = sched_index(c, t0 + (uint32_t)i);
+        if (si >= 0)
+            hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c->stream, A->swork, c->sched_dt + si,
+                               c->sched_c + si);
+        hipLaunchKernelGGL(k_copy_scalars, dim3(1), dim3(1), 0, c->stream, A->swork, A->hist + i);
+        StepArgs a = make_args(c);
+        a.scal = A->swork;
+        a.visc = nullptr;
+        a.xp = A->tx + (size_t)i * 3 * N;
+        a.xc = A->tx + (size_t)(i + 1) * 3 * N;
+        a.xn = A->tx + (size_t)(i + 2) * 3 * N;
+        a.v = A->tv + (size_t)i * 3 * N;
+        a.v_out = A->tv + (size_t)(i + 1) * 3 * N;
+        a.stress = A->ts + (size_t)i * 60 * E;
+        a.stress_out = A->ts + (size_t)(i + 1) * 60 * E;
+        a.ps = A->tp ? A->tp + (size_t)i * 10 * E : nullptr;
+        a.ps_out = A->tp ? A->tp + (size_t)(i + 1) * 10 * E : nullptr;
+        if ((rc = launch_elem_args(c, a))) break;
+        StepArgs an = a;
+        an.stress = a.stress_out;
+        an.tape_f = A->tf + (size_t)i * 3 * N;
+        an.tape_m = A->tm + (size_t)i * N;
+        an.tape_code = A->tc + (size_t)i * N;
+        if (si >= 0 && c->sched_damp) an.damp_sched = c->sched_damp + (size_t)si * N;
+        if (an.mat)
+            hipLaunchKernelGGL(k_node<true>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, an);
+        else
+            hipLaunchKernelGGL(k_node<false>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, an);
+        StepArgs ag = a;
+        ag.xc = a.xn;
+        hipLaunchKernelGGL(k_geo, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, ag);
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, ag, c->n_geo_blocks, c->n_node_blocks);
+        if (hipGetLastError() != hipSuccess) rc = set_err(SNH_EHIP, "adjoint tape step launch failed");
+    }
+    return rc;
+}
+
+/* copy a state (stress, plastic strain, x, x_prev, v, scalars) between the
+   context, a checkpoint or a tape slot */
+static int adj_copy_state(snh_context* c, double* ds, double* dp, double* dx, double* dxp, double* dv,
+                          DevScalars* dsc, const double* ss, const double* sp, const double* sx, const double* sxp,
+                          const double* sv, const DevScalars* ssc) {
+    const size_t E = c->E, N = c->N;
+    HIPCHK(hipMemcpyAsync(ds, ss, 60 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (dp && sp) HIPCHK(hipMemcpyAsync(dp, sp, 10 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dx, sx, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dxp, sxp, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dv, sv, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_copy_scalars, dim3(1), dim3(1), 0, c->stream, ssc, dsc);
+    HIPCHK(hipGetLastError());
+    return SNH_OK;
+}
+
+/* checkpoint of the state entering timeStep t (sources: the context's
+   buffers or a tape slot) */
+static int adj_save_checkpoint(snh_context* c, AdjointCtx* A, uint32_t t, const double* ss, const double* sp,
+                               const double* sx, const double* sxp, const double* sv, const DevScalars* ssc) {
     AdjCheckpoint k;
     const size_t E = c->E, N = c->N;
-    k.timeStep = c->timeStep;
+    k.timeStep = t;
     bool ok = hipMalloc((void**)&k.stress, 60 * E * sizeof(double)) == hipSuccess &&
               (!c->ps || hipMalloc((void**)&k.ps, 10 * E * sizeof(double)) == hipSuccess) &&
               hipMalloc((void**)&k.x, 3 * N * sizeof(double)) == hipSuccess &&
@@ -144,14 +209,7 @@ static int adj_save_checkpoint(snh_context* c, AdjointCtx* A) {
               hipMalloc((void**)&k.scal, sizeof(DevScalars)) == hipSuccess;
     A->ckpt.push_back(k);   /* freed with the context's adjoint either way */
     if (!ok) return set_err(SNH_ENOMEM, "adjoint: cannot allocate checkpoint %zu", A->ckpt.size());
-    HIPCHK(hipMemcpyAsync(k.stress, c->stress, 60 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (c->ps) HIPCHK(hipMemcpyAsync(k.ps, c->ps, 10 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(k.x, c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(k.xp, c->x[c->cur ^ 1], 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(k.v, c->v, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    hipLaunchKernelGGL(k_copy_scalars, dim3(1), dim3(1), 0, c->stream, c->scal, k.scal);
-    HIPCHK(hipGetLastError());
-    return SNH_OK;
+    return adj_copy_state(c, k.stress, k.ps, k.x, k.xp, k.v, k.scal, ss, sp, sx, sxp, sv, ssc);
 }
 
 extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
@@ -159,19 +217,53 @@ extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
     AdjointCtx* A = c->adj;
     if (!A) return set_err(SNH_ESTATE, "snh_adjoint_forward before snh_adjoint_begin");
     if (nsteps < 0) return set_err(SNH_EINVAL, "negative step count");
+    if (nsteps == 0) return SNH_OK;
+    if (A->steps > 0 && c->timeStep != A->ckpt[0].timeStep + (uint32_t)A->steps)
+        return set_err(SNH_ESTATE, "steps were run outside snh_adjoint_forward since snh_adjoint_begin");
+    const size_t E = c->E, N = c->N;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     (void)hipEventRecord(e0, c->stream);
     int rc = SNH_OK;
     int32_t done = 0;
+    /* the steps run in tape mode, so the last segment needs no replay; the
+       current state is in the context's buffers at entry and in tape slot
+       tape_len afterwards */
+    bool in_tape = false;
     while (done < nsteps && rc == SNH_OK) {
-        if (A->steps % A->K == 0) rc = adj_save_checkpoint(c, A);
-        if (rc) break;
+        if (A->steps % A->K == 0) {
+            const int L = A->tape_len;
+            const double* ss = in_tape ? A->ts + (size_t)L * 60 * E : c->stress;
+            const double* sp = in_tape ? (A->tp ? A->tp + (size_t)L * 10 * E : nullptr) : c->ps;
+            const double* sx = in_tape ? A->tx + (size_t)(L + 1) * 3 * N : c->x[c->cur];
+            const double* sxp = in_tape ? A->tx + (size_t)L * 3 * N : c->x[c->cur ^ 1];
+            const double* sv = in_tape ? A->tv + (size_t)L * 3 * N : c->v;
+            const DevScalars* ssc = in_tape ? A->swork : c->scal;
+            if ((rc = adj_save_checkpoint(c, A, c->timeStep, ss, sp, sx, sxp, sv, ssc))) break;
+            const AdjCheckpoint& ck = A->ckpt.back();
+            rc = adj_copy_state(c, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv, A->swork, ck.stress, ck.ps, ck.x,
+                                ck.xp, ck.v, ck.scal);
+            if (rc) break;
+            A->tape_seg = A->steps / A->K;
+            A->tape_len = 0;
+        }
+        in_tape = true;
         const int32_t todo = std::min(nsteps - done, A->K - A->steps % A->K);
-        rc = snh_step(c, todo);
+        const uint32_t t0 = A->ckpt[A->tape_seg].timeStep;
+        if ((rc = adj_run_tape(c, A, t0, A->tape_len, A->tape_len + todo))) break;
+        A->tape_len += todo;
         done += todo;
         A->steps += todo;
+        c->timeStep += todo;
+    }
+    if (rc == SNH_OK) {
+        const int L = A->tape_len;
+        rc = adj_copy_state(c, c->stress, c->ps, c->x[c->cur], c->x[c->cur ^ 1], c->v, c->scal,
+                            A->ts + (size_t)L * 60 * E, A->tp ? A->tp + (size_t)L * 10 * E : nullptr,
+                            A->tx + (size_t)(L + 1) * 3 * N, A->tx + (size_t)L * 3 * N, A->tv + (size_t)L * 3 * N,
+                            A->swork);
+        c->rotation_pending = true;
     }
     (void)hipEventRecord(e1, c->stream);
     if (rc == SNH_OK && hipEventSynchronize(e1) == hipSuccess) {
@@ -237,46 +329,14 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
         const int L = std::min(K, A->steps - s0);
         const AdjCheckpoint& ck = A->ckpt[seg];
         (void)hipEventRecord(ev[3 * seg], c->stream);
-        /* restore the checkpoint into slot 0 */
-        HIPCHK(hipMemcpyAsync(A->ts, ck.stress, 60 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        if (A->tp) HIPCHK(hipMemcpyAsync(A->tp, ck.ps, 10 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(A->tx, ck.xp, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(A->tx + 3 * N, ck.x, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(A->tv, ck.v, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        hipLaunchKernelGGL(k_copy_scalars, dim3(1), dim3(1), 0, c->stream, ck.scal, A->swork);
-        /* replay */
-        for (int i = 0; i < L && rc == SNH_OK; i++) {
-            const int si = sched_index(c, ck.timeStep + (uint32_t)i);
-            if (si >= 0)
-                hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c->stream, A->swork, c->sched_dt + si,
-                                   c->sched_c + si);
-            hipLaunchKernelGGL(k_copy_scalars, dim3(1), dim3(1), 0, c->stream, A->swork, A->hist + i);
-            StepArgs a = make_args(c);
-            a.scal = A->swork;
-            a.visc = nullptr;
-            a.xp = A->tx + (size_t)i * 3 * N;
-            a.xc = A->tx + (size_t)(i + 1) * 3 * N;
-            a.xn = A->tx + (size_t)(i + 2) * 3 * N;
-            a.v = A->tv + (size_t)i * 3 * N;
-            a.v_out = A->tv + (size_t)(i + 1) * 3 * N;
-            a.stress = A->ts + (size_t)i * 60 * E;
-            a.stress_out = A->ts + (size_t)(i + 1) * 60 * E;
-            a.ps = A->tp ? A->tp + (size_t)i * 10 * E : nullptr;
-            a.ps_out = A->tp ? A->tp + (size_t)(i + 1) * 10 * E : nullptr;
-            if ((rc = launch_elem_args(c, a))) break;
-            StepArgs an = a;
-            an.stress = a.stress_out;
-            an.tape_f = A->tf + (size_t)i * 3 * N;
-            an.tape_m = A->tm + (size_t)i * N;
-            an.tape_code = A->tc + (size_t)i * N;
-            if (si >= 0 && c->sched_damp) an.damp_sched = c->sched_damp + (size_t)si * N;
-            hipLaunchKernelGGL(k_node, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, an);
-            StepArgs ag = a;
-            ag.xc = a.xn;
-            hipLaunchKernelGGL(k_geo, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, ag);
-            hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, ag, c->n_geo_blocks,
-                               c->n_node_blocks);
-            if (hipGetLastError() != hipSuccess) rc = set_err(SNH_EHIP, "adjoint replay launch failed");
+        /* the tape still holds the last segment recorded; others are
+           replayed from their checkpoint */
+        if (!(A->tape_seg == seg && A->tape_len == L)) {
+            rc = adj_copy_state(c, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv, A->swork, ck.stress, ck.ps, ck.x,
+                                ck.xp, ck.v, ck.scal);
+            if (rc == SNH_OK) rc = adj_run_tape(c, A, ck.timeStep, 0, L);
+            A->tape_seg = seg;
+            A->tape_len = L;
         }
         (void)hipEventRecord(ev[3 * seg + 1], c->stream);
         /* reverse */
@@ -291,6 +351,7 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
             na.tape_m = A->tm + (size_t)i * N;
             na.tape_code = A->tc + (size_t)i * N;
             na.bc = c->bc;
+            na.bcs = c->dbc;
             na.scal = A->hist + i;
             na.demf = c->cfg.demf;
             na.I = c->I; na.J = c->J; na.K = c->K;
@@ -313,6 +374,7 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
             aa.grad = A->grad;
             aa.nb = A->nb;
             aa.param = A->cfg.param;
+            aa.flags = A->cfg.flags;
             rc = launch_elem_adj(c, aa);
             have_nb = true;
         }

@@ -52,8 +52,13 @@ struct DevMaterial {
     /* plasticity */
     uint32_t nseg;
     double plstrain[9], fa[9], da[9], coh[9];
+    /* slopes of each softening segment, (x[i+1] - x[i]) / (pl[i+1] - pl[i]),
+       computed once on the host (the same correctly rounded quotients) */
+    double tgf[8], tgd[8], tgc[8];
     double ten_off;
     double sphi, spsi, tanphi;  /* host libm sin/sin/tan of fa[0], da[0] */
+    double anphi0, sqanphi0;    /* (1 + sphi) / (1 - sphi) and its root, for the
+                                   elastic bound only */
     /* viscosity */
     double vis_min, vis_max, refvisc, refsrate, reftemp, activationE, srexponent;
     int linear_visc;         /* srexponent == 1 and activationE == 0 */
@@ -100,6 +105,8 @@ struct StepArgs {
        sign(v) is replaced by code * sign(f), which gives the same bits when
        the code is the one the step would compute */
     const uint8_t* damp_sched;
+    const DevMaterial* __restrict__ mats;  /* material table of this context */
+    const DevBC* __restrict__ bcs;         /* velocity-condition table        */
     double* __restrict__ part_min;
     double* __restrict__ part_vmax;
     DevScalars* __restrict__ scal;

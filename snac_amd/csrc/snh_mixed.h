@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
     const double dt = a.scal->dt;
     const float dtf = (float)dt;
     const float dt_rotf = (float)a.scal->dt_rot;
-    const DevMaterial& m = c_mat[a.mat ? a.mat[ec] : 0];
+    const DevMaterial& m = a.mats[a.mat ? a.mat[ec] : 0];
 
     int nd[4];
     double X0[3];
@@ -170,9 +170,9 @@ __global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
             for (uint32_t i = 0; i < m.nseg; i++) {
                 const double pl1 = m.plstrain[i], pl2 = m.plstrain[i + 1];
                 if (ps >= pl1 && ps <= pl2) {
-                    const double tgf = (m.fa[i + 1] - m.fa[i]) / (pl2 - pl1);
-                    const double tgd = (m.da[i + 1] - m.da[i]) / (pl2 - pl1);
-                    const double tgc = (m.coh[i + 1] - m.coh[i]) / (pl2 - pl1);
+                    const double tgf = m.tgf[i];
+                    const double tgd = m.tgd[i];
+                    const double tgc = m.tgc[i];
                     own.phi = m.fa[i] + tgf * (ps - pl1);
                     own.psi = m.da[i] + tgd * (ps - pl1);
                     own.coh = m.coh[i] + tgc * (ps - pl1);
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
         if (sp.phi > 0.0 && tc == 0.0) tc = coh / tan_phi(m, sp.phi);
         double sphi, spsi, dps = 0.0;
         angle_terms(m, sp.phi, sp.psi, sphi, spsi);
-        if (!mc_surely_elastic(st, sphi, coh, tc)) {
+        if (!mc_surely_elastic(m, st, sphi, coh, tc)) {
             double d3[3], cn[3][3];
             principal_stresses(S, d3, cn);
             if (mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dps)) {
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
             const int ei = i - 1 + (slot & 1), ej = j - 1 + ((slot >> 1) & 1), ek = k - 1 + ((slot >> 2) & 1);
             if (ei < 0 || ej < 0 || ek < 0 || ei >= a.EI || ej >= a.EJ || ek >= a.EK) continue;
             const int e = ei + a.EI * (ej + a.EJ * ek);
-            const DevMaterial& m = c_mat[a.mat ? a.mat[e] : 0];
+            const DevMaterial& m = a.mats[a.mat ? a.mat[e] : 0];
             double X[8][3];
             load_hex(a, a.xc, ei, ej, ek, X);
             /* reference stress of this element: its first listed tet */
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
         v2 += dt * f2 / im;
         const uint32_t code = a.bc[n];
         if (code) {
-            const DevBC& b = c_bc[code];
+            const DevBC& b = a.bcs[code];
             if (b.mask & 1u) v0 = b.val[0];
             if (b.mask & 2u) v1 = b.val[1];
             if (b.mask & 4u) v2 = b.val[2];

@@ -1,0 +1,74 @@
+"""Ensembles: several independent models stepped together by one HIP graph
+(snh_ensemble_* in the C ABI).  Each member keeps its own context, stream,
+materials, BCs and element-pass variant (chosen per member by the
+autotuner); a graph launch advances all of them graph_steps steps, members on
+parallel branches.
+
+    members = [Context(p).Construct().Build().Initialise() for p in problems]
+    ens = Ensemble(members, graph_steps=10, autotune=True)
+    ens.step(1000)
+    ens.synchronize()
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List
+
+import numpy as np
+
+from . import _abi
+from .context import Context
+from .problem import Problem, mohr_coulomb_extension
+
+
+class Ensemble:
+    def __init__(self, members: List[Context], graph_steps: int = 10, autotune: bool = True):
+        self.members = members
+        self.lib = _abi.load_library()
+        if autotune:
+            self.tuning = [m.autotune() for m in members]
+        arr = (C.c_void_p * len(members))(*[m.h.value for m in members])
+        h = C.c_void_p()
+        _abi.check(self.lib.snh_ensemble_create(arr, len(members), int(graph_steps), C.byref(h)), self.lib)
+        self.h = h
+
+    def step(self, nsteps: int):
+        _abi.check(self.lib.snh_ensemble_step(self.h, int(nsteps)), self.lib)
+        return self
+
+    def synchronize(self):
+        _abi.check(self.lib.snh_ensemble_synchronize(self.h), self.lib)
+        return self
+
+    def destroy(self):
+        if self.h is not None:
+            self.lib.snh_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def elastoplastic_ensemble(n_models=64, seed=5, nx=(16, 96), ny=(8, 32), phi=(20.0, 35.0),
+                           cohesion=(10.0e6, 50.0e6)) -> List[Problem]:
+    """Elasto-plastic extension models of uneven sizes: (nx, nz) uniform in
+    nx, ny uniform in ny (elements; element size 2 km x 2 km x 2 km), friction
+    angle uniform in phi, initial cohesion uniform in `cohesion` softening to
+    a tenth over plastic strain 0..0.5, all from one seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_models):
+        ex, ez = (int(v) for v in rng.integers(nx[0], nx[1] + 1, size=2))
+        ey = int(rng.integers(ny[0], ny[1] + 1))
+        f = float(rng.uniform(*phi))
+        c0 = float(rng.uniform(*cohesion))
+        p = mohr_coulomb_extension(elements=(ex, ey, ez), extent_km=(2.0 * ex, 2.0 * ey, 2.0 * ez), seed=k + 1)
+        m = p.materials[0]
+        m.frictionAngle = (f, f, f)
+        m.cohesion = (c0, 0.1 * c0, 0.1 * c0)
+        p.name = f"elastoplastic_{k}"
+        out.append(p)
+    return out

@@ -298,9 +298,25 @@ def mohr_coulomb_shear(elements=(64, 32, 64), extent_km=(64.0, 32.0, 64.0), seed
     return p
 
 
+def mohr_coulomb_weak_crust(elements=(64, 32, 64), extent_km=(400.0, 100.0, 200.0), seed=1) -> Problem:
+    """The Mohr-Coulomb extension with a weak crust, for the cohesion
+    adjoint: cohesion 0.1 MPa softening to 0.05 MPa over plastic strain
+    0..0.5 (friction 30 deg, dilation 0) and strain_inert 1e-3, so that the
+    top layers yield within a few hundred steps.  Gravity 10, hydrostatic
+    initial stress, x-faces pulled apart at 1 cm/yr."""
+    p = mohr_coulomb_extension(elements=elements, extent_km=extent_km, seed=seed)
+    m = p.materials[0]
+    m.cohesion = (1.0e5, 5.0e4, 5.0e4)
+    m.plstrain = (0.0, 0.5, 1.0e3)
+    p.strain_inert = 1.0e-3
+    p.name = "mohr_coulomb_weak_crust"
+    return p
+
+
 WORKLOADS = {
     "elastic_box_compression": elastic_box_compression,
     "mohr_coulomb_extension": mohr_coulomb_extension,
     "maxwell_viscoelastic": maxwell_viscoelastic,
     "mohr_coulomb_shear": mohr_coulomb_shear,
+    "mohr_coulomb_weak_crust": mohr_coulomb_weak_crust,
 }

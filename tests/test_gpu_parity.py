@@ -10,7 +10,8 @@ TOL = 1e-12   # FP64 parity bar: relative to the field's max magnitude
 pytestmark = pytest.mark.gpu
 
 # both element-pass variants: one tet per lane, one element per lane
-VARIANTS = pytest.mark.parametrize("variant", [0, 1], ids=["tet_lane", "elem_lane"])
+VARIANTS = pytest.mark.parametrize("variant", [0, 1, 2, 3],
+                                   ids=["tet_lane", "elem_lane", "elem_lane_lowreg", "tet_wave"])
 
 
 def _report(name, ref, got, fields=FIELDS):
@@ -56,6 +57,23 @@ def test_mohr_coulomb_shear_yielding(variant):
     _check(f"mc_shear[{variant}]", ref, got, rsc, gsc)
     ps = ref["plastic_strain"]
     assert np.sum((ps > 0.0) & (ps != 0.5)) > 100   # tets that yielded during the run
+
+
+@VARIANTS
+def test_two_materials_small(variant):
+    """Per-element material ids: a Mohr-Coulomb lower half under an elastic
+    upper half with other moduli and density."""
+    p = P.mohr_coulomb_shear(elements=(8, 6, 6))
+    from snac_amd import _abi
+    upper = P.MaterialSpec(rheology=_abi.RHEOLOGY_ELASTIC, alpha=0.0, beta=0.0, lam=2.0e10, mu=2.5e10,
+                           density=2500.0)
+    p.materials = [p.materials[0], upper]
+    ni, nj, nk = p.elements
+    mat = np.zeros((nk, nj, ni), dtype=np.uint8)
+    mat[:, nj // 2:, :] = 1
+    p.element_material = mat.reshape(-1)
+    ref, got, rsc, gsc = run_both(p, 30, variant=variant)
+    _check(f"two_materials[{variant}]", ref, got, rsc, gsc)
 
 
 @VARIANTS
@@ -128,6 +146,7 @@ def test_autotune_keeps_state_and_results():
     rep = a.autotune()
     print(f"  autotune: {rep}", flush=True)
     assert rep["elem_ms_tet_per_lane"] > 0 and rep["elem_ms_element_per_lane"] > 0
+    assert rep["elem_ms_element_per_lane_lowreg"] > 0 and rep["elem_ms_tet_per_wave"] > 0
     a.Execute(20)
     sa = a.state()
     a.Destroy()
