@@ -251,6 +251,25 @@ int snh_autotune(snh_context* ctx, snh_tune_report* out);
 /* Force one element-pass variant (SNH_ELEM_*); both give identical results. */
 int snh_set_elem_variant(snh_context* ctx, int32_t variant);
 
+/* ---- dump diagnostics -----------------------------------------------------
+ * While enabled, every step also records what the reference's dumps hold
+ * besides the state (Context.c:1311-1489, plastic_BI/Output.c:60-80): the
+ * element strain-rate invariant (StrainRate.c:272-304), stress invariant and
+ * hydrostatic pressure (Stress.c:140-156), the volume-averaged plastic strain
+ * (plastic_BI/Constitutive.c:232-236) and the damped nodal force
+ * (UpdateNode.c:102-104).  Enable it for the steps that end on a dump
+ * (FP64 contexts only; SNH_EUNSUPPORTED for mixed precision). */
+typedef struct snh_diagnostics {
+    double* strain_rate;      /* [E] or NULL */
+    double* stress;           /* [E] or NULL */
+    double* hydro_pressure;   /* [E] or NULL */
+    double* plastic_strain;   /* [E] or NULL (0 for laws without one) */
+    double* force;            /* [N][3] or NULL */
+} snh_diagnostics;
+int snh_set_diagnostics(snh_context* ctx, int enable);
+/* values of the last step run with diagnostics enabled (host arrays) */
+int snh_get_diagnostics(snh_context* ctx, const snh_diagnostics* out);
+
 /* Prescribe, for the next nsteps calls of the step (snh_step; FP64 path),
  * dt and speed of sound of each step and optionally the damping branch
  * sign(v) sign(f) of every node component (damp: [nsteps][N][3] in

@@ -131,6 +131,12 @@ class TuneReport(C.Structure):
     _fields_ = [("elem_ms", C.c_double * ELEM_VARIANTS), ("elem_variant", C.c_int32)]
 
 
+class Diagnostics(C.Structure):
+    _fields_ = [("strain_rate", C.POINTER(C.c_double)), ("stress", C.POINTER(C.c_double)),
+                ("hydro_pressure", C.POINTER(C.c_double)), ("plastic_strain", C.POINTER(C.c_double)),
+                ("force", C.POINTER(C.c_double))]
+
+
 PARAM_LOG_VISCOSITY = 0
 PARAM_COHESION = 1
 
@@ -173,6 +179,8 @@ SIGNATURES = {
     "snh_autotune": (_I, [_P, C.POINTER(TuneReport)]),
     "snh_set_elem_variant": (_I, [_P, C.c_int32]),
     "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
+    "snh_set_diagnostics": (_I, [_P, _I]),
+    "snh_get_diagnostics": (_I, [_P, C.POINTER(Diagnostics)]),
     "snh_set_schedule": (_I, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_int8)]),
     "snh_ensemble_create": (_I, [C.POINTER(_P), C.c_int32, C.c_int32, C.POINTER(_P)]),

@@ -118,6 +118,22 @@ class Context:
         self._check(self.lib.snh_set_elem_variant(self.h, int(variant)))
         return self
 
+    def set_diagnostics(self, enable: bool = True):
+        """Record the reference's dump quantities in the following steps."""
+        self._check(self.lib.snh_set_diagnostics(self.h, int(enable)))
+        return self
+
+    def diagnostics(self) -> Dict[str, np.ndarray]:
+        """Element strain-rate and stress invariants, hydrostatic pressure,
+        volume-averaged plastic strain [E] and damped nodal force [N, 3] of
+        the last step run with diagnostics on."""
+        out = {k: np.zeros(self.E) for k in ("strain_rate", "stress", "hydro_pressure", "plastic_strain")}
+        out["force"] = np.zeros((self.N, 3))
+        d = _abi.Diagnostics(*[_dp(out[k]) for k in ("strain_rate", "stress", "hydro_pressure", "plastic_strain",
+                                                      "force")])
+        self._check(self.lib.snh_get_diagnostics(self.h, C.byref(d)))
+        return out
+
     def set_schedule(self, dt: np.ndarray, speed_of_sound: np.ndarray, damp: Optional[np.ndarray] = None):
         """Prescribe dt, speed of sound and (optionally) the damping branch
         ([n, N, 3] int8 codes sign(v) sign(f)) of the next n steps."""

@@ -829,6 +829,22 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
     evol *= 0.5;                        /* UpdateElement.c:267-268 */
     const double elemVolume = 2.0 * evol; /* StrainRate.c:57, Stress.c:68 */
     const double srcorr = partial / elemVolume;
+    if constexpr (!LOWREG) {
+        if (a.diag) {
+            /* element strain-rate invariant (StrainRate.c:272-304) */
+            double srVolAvg = 0.0, srOtherAvg = 0.0;
+#pragma unroll
+            for (int t = 0; t < NTET; t++) {
+                const double corr = -1.0 * trace[t] + partial / elemVolume;
+                const double s00 = sr[t][0] + corr, s11 = sr[t][1] + corr, s22 = sr[t][2] + corr;
+                srVolAvg += s11 * s22 + s22 * s00 + s00 * s11;
+                srOtherAvg += sr[t][3] * sr[t][3] + sr[t][5] * sr[t][5] + sr[t][4] * sr[t][4];
+            }
+            srVolAvg /= (double)NTET;
+            srOtherAvg /= (double)NTET;
+            a.diag[e] = 0.5 * sqrt(0.5 * fabs(-1.0 * srVolAvg + srOtherAvg));
+        }
+    }
 
     /* pass 2: rotation, strain, constitutive */
     double sig[LOWREG ? 1 : NTET][6];
@@ -840,6 +856,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
     double pphi = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC) ? 30.0 : 0.0;
     double ppsi = 0.0, phard = 0.0, ptc = 0.0;
     const double cohoff = a.cohoff ? a.cohoff[e] : 0.0;
+    double depls = 0.0, totalVolume = 0.0;   /* aps (plastic_BI/Constitutive.c:232-236) */
 #pragma unroll
     for (int t = 0; t < NTET; t++) {
         TetKin kr;
@@ -915,6 +932,8 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             /* in place, an unchanged value need not be stored */
             if (a.ps_out != a.ps || plasticStrain != a.ps[(size_t)t * E + e])
                 a.ps_out[(size_t)t * E + e] = plasticStrain;
+            depls += plasticStrain * volt;
+            totalVolume += volt;
         } else if ((RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)) {
             double plasticStrain = a.ps[(size_t)t * E + e];
             double trace_strain = sn[0] + sn[1] + sn[2];
@@ -951,6 +970,8 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             /* in place, an unchanged value need not be stored */
             if (a.ps_out != a.ps || plasticStrain != a.ps[(size_t)t * E + e])
                 a.ps_out[(size_t)t * E + e] = plasticStrain;
+            depls += plasticStrain * volt;
+            totalVolume += volt;
         } else if ((RSET & RS_MAXWELL) && (m.rheology & SNH_RHEOLOGY_MAXWELL)) {
             /* trace strain from the volume ratio (maxwell/Constitutive.c:113) */
             double Xp[4][3];
@@ -1011,6 +1032,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             pst += tr[t] * vol[t];
         }
         const double avgTrace = pst / elemVolume;
+        double sVolAvg = 0.0, sOtherAvg = 0.0, pressure = 0.0;
 #pragma unroll
         for (int t = 0; t < NTET; t++) {
             sig[t][0] += -1.0 * tr[t] + avgTrace;
@@ -1018,6 +1040,21 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             sig[t][2] += -1.0 * tr[t] + avgTrace;
 #pragma unroll
             for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + e] = sig[t][c];
+            if (a.diag) {
+                /* element stress invariant and pressure (Stress.c:140-156) */
+                const double s00 = sig[t][0] - avgTrace, s11 = sig[t][1] - avgTrace, s22 = sig[t][2] - avgTrace;
+                sVolAvg += s11 * s22 + s22 * s00 + s00 * s11;
+                sOtherAvg += sig[t][3] * sig[t][3] + sig[t][5] * sig[t][5] + sig[t][4] * sig[t][4];
+                pressure += snh_div3(sig[t][0] + sig[t][1] + sig[t][2]);
+            }
+        }
+        if (a.diag) {
+            sVolAvg /= (double)NTET;
+            sOtherAvg /= (double)NTET;
+            pressure /= (double)NTET;
+            a.diag[E + e] = 0.5 * sqrt(0.5 * fabs(-1.0 * sVolAvg + sOtherAvg));
+            a.diag[2 * E + e] = -1.0 * pressure;
+            a.diag[3 * E + e] = totalVolume > 0.0 ? depls / totalVolume : 0.0;
         }
     }
 }
@@ -1161,6 +1198,11 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
         f0 -= a.demf * sv0 * fabs(f0);
         f1 -= a.demf * sv1 * fabs(f1);
         f2 -= a.demf * sv2 * fabs(f2);
+        if (a.force_out) {
+            a.force_out[n] = f0;
+            a.force_out[N + n] = f1;
+            a.force_out[2 * N + n] = f2;
+        }
         v0 += dt * f0 / im;
         v1 += dt * f1 / im;
         v2 += dt * f2 / im;
@@ -1338,6 +1380,9 @@ struct snh_context {
     uint32_t timeStep;
     int elem_variant;   /* SNH_ELEM_* */
     struct AdjointCtx* adj;   /* recorded adjoint run, or null */
+    bool diag_on;             /* steps record dump diagnostics */
+    double* diag_buf;         /* [5][E] */
+    double* force_buf;        /* [3][N] */
     /* prescribed dt / speed of sound / damping branch for steps
        [sched_base, sched_base + sched_n) of timeStep */
     double* sched_dt;
@@ -1454,6 +1499,8 @@ static StepArgs make_args(snh_context* c) {
     a.tape_m = nullptr;
     a.tape_code = nullptr;
     a.damp_sched = nullptr;
+    a.diag = c->diag_on ? c->diag_buf : nullptr;
+    a.force_out = c->diag_on ? c->force_buf : nullptr;
     a.visc = c->visc;
     a.mat = c->mat;
     a.refvisc = c->refvisc;
@@ -1477,7 +1524,7 @@ static StepArgs make_args(snh_context* c) {
 static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
                     c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
-                    c->dmat, c->dbc};
+                    c->dmat, c->dbc, c->diag_buf, c->force_buf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1668,6 +1715,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     c->cur = 0;
     c->elem_variant = SNH_ELEM_TET_PER_LANE;
     c->adj = nullptr;
+    c->diag_on = false;
+    c->diag_buf = c->force_buf = nullptr;
     c->sched_dt = c->sched_c = nullptr;
     c->sched_damp = nullptr;
     c->sched_base = 0;
@@ -1799,20 +1848,32 @@ static int launch_elem_args(snh_context* c, StepArgs a) {
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
     const bool mm = a.mat != nullptr;
     if (c->cfg.precision == SNH_PRECISION_MIXED) {
-        switch (c->rset) {
-        case RS_ELASTIC: hipLaunchKernelGGL(k_elem_mx<RS_ELASTIC>, grid, block, 0, c->stream, a); break;
-        case RS_PLASTIC: hipLaunchKernelGGL(k_elem_mx<RS_PLASTIC>, grid, block, 0, c->stream, a); break;
-        case RS_MAXWELL: hipLaunchKernelGGL(k_elem_mx<RS_MAXWELL>, grid, block, 0, c->stream, a); break;
-        default:
-            hipLaunchKernelGGL(k_elem_mx<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL>, grid, block, 0, c->stream, a);
-            break;
+        /* tet per wave unless the tet-per-lane layout is forced */
+        const bool tw = c->elem_variant != SNH_ELEM_TET_PER_LANE;
+        if (tw) {
+            grid = dim3((c->E + 63) / 64);
+            block = dim3(640);
         }
+#define MX_LAUNCH(R)                                                                          \
+    do {                                                                                      \
+        if (tw) hipLaunchKernelGGL((k_elem_mx<R, true>), grid, block, 0, c->stream, a);       \
+        else hipLaunchKernelGGL((k_elem_mx<R, false>), grid, block, 0, c->stream, a);         \
+    } while (0)
+        switch (c->rset) {
+        case RS_ELASTIC: MX_LAUNCH(RS_ELASTIC); break;
+        case RS_PLASTIC: MX_LAUNCH(RS_PLASTIC); break;
+        case RS_MAXWELL: MX_LAUNCH(RS_MAXWELL); break;
+        default: MX_LAUNCH(RS_ELASTIC | RS_PLASTIC | RS_MAXWELL); break;
+        }
+#undef MX_LAUNCH
         HIPCHK(hipGetLastError());
         return SNH_OK;
     }
-    if (c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE || c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG) {
+    /* the diagnostics are written by the element-per-lane variant only */
+    if (a.diag || c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE ||
+        c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG) {
         dim3 g2((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), b2(ELEM_BLOCK);
-        const bool lr = c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG;
+        const bool lr = !a.diag && c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG;
 #define SEQ_LAUNCH(R)                                                                         \
     do {                                                                                      \
         if (mm) {                                                                             \
@@ -1990,6 +2051,37 @@ extern "C" int snh_step(snh_context* c, int nsteps) {
         c->timeStep++;
     }
     c->rotation_pending = c->rotation_pending || nsteps > 0;
+    return SNH_OK;
+}
+
+extern "C" int snh_set_diagnostics(snh_context* c, int enable) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (enable && c->cfg.precision != SNH_PRECISION_FP64)
+        return set_err(SNH_EUNSUPPORTED, "dump diagnostics are written by the FP64 path");
+    if (enable && !c->diag_buf) {
+        HIPCHK(hipMalloc((void**)&c->diag_buf, 5 * (size_t)c->E * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c->force_buf, 3 * (size_t)c->N * sizeof(double)));
+        HIPCHK(hipMemset(c->diag_buf, 0, 5 * (size_t)c->E * sizeof(double)));
+        HIPCHK(hipMemset(c->force_buf, 0, 3 * (size_t)c->N * sizeof(double)));
+    }
+    c->diag_on = enable != 0;
+    return SNH_OK;
+}
+
+extern "C" int snh_get_diagnostics(snh_context* c, const snh_diagnostics* out) {
+    if (!c || !out) return set_err(SNH_EINVAL, "null argument");
+    if (!c->diag_buf) return set_err(SNH_ESTATE, "no diagnostics recorded (snh_set_diagnostics)");
+    const size_t E = c->E, N = c->N;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double* dst[4] = {out->strain_rate, out->stress, out->hydro_pressure, out->plastic_strain};
+    for (int k = 0; k < 4; k++)
+        if (dst[k]) HIPCHK(hipMemcpy(dst[k], c->diag_buf + k * E, E * sizeof(double), hipMemcpyDeviceToHost));
+    if (out->force) {
+        std::vector<double> f(3 * N);
+        HIPCHK(hipMemcpy(f.data(), c->force_buf, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) out->force[3 * n + d] = f[d * N + n];
+    }
     return SNH_OK;
 }
 

@@ -105,6 +105,12 @@ struct StepArgs {
        sign(v) is replaced by code * sign(f), which gives the same bits when
        the code is the one the step would compute */
     const uint8_t* damp_sched;
+    /* dump diagnostics of this step, or null: element scalars [5][E]
+       (strain-rate invariant, stress invariant, hydrostatic pressure,
+       volume-averaged plastic strain, unused) and the damped nodal force
+       [3][N] */
+    double* diag;
+    double* force_out;
     const DevMaterial* __restrict__ mats;  /* material table of this context */
     const DevBC* __restrict__ bcs;         /* velocity-condition table        */
     double* __restrict__ part_min;

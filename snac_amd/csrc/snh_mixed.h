@@ -30,54 +30,42 @@ struct TetKinF {
     float w[3];
 };
 
-/* E: tet node coordinates relative to its node 0 (E[0] = 0), U: velocities */
+/* E: tet node coordinates relative to its node 0 (E[0] = 0), U: velocities.
+   Uses n A = a / 2 with a the face cross product (the identity behind
+   StrainRate.c:65-184), so no face needs a square root or a division:
+   L = -(1 / 6V) sum_k u_k (x) a_(k+1)%4, strain rate sym(L), spin skew(L). */
 __device__ __forceinline__ void tet_kinematics_f(const float E[4][3], const float U[4][3], TetKinF& o) {
 #pragma clang fp contract(fast)
-    float n[4][3], A[4];
-    float a0 = 0.f, b0 = 0.f, c0 = 0.f;
+    float a[4][3];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const float* p0 = E[kFaceToNode[f][0]];
         const float* p1 = E[kFaceToNode[f][1]];
         const float* p2 = E[kFaceToNode[f][2]];
-        float x2 = p1[0] - p0[0], y2 = p1[1] - p0[1], z2 = p1[2] - p0[2];
-        float x3 = p2[0] - p0[0], y3 = p2[1] - p0[1], z3 = p2[2] - p0[2];
-        float a = y2 * z3 - z2 * y3, b = z2 * x3 - x2 * z3, c = x2 * y3 - y2 * x3;
-        if (f == 0) {
-            a0 = a;
-            b0 = b;
-            c0 = c;
-        }
-        float s = __builtin_sqrtf(a * a + b * b + c * c);
-        float is = 1.0f / s;
-        A[f] = 0.5f * s;
-        n[f][0] = a * is;
-        n[f][1] = b * is;
-        n[f][2] = c * is;
+        const float x2 = p1[0] - p0[0], y2 = p1[1] - p0[1], z2 = p1[2] - p0[2];
+        const float x3 = p2[0] - p0[0], y3 = p2[1] - p0[1], z3 = p2[2] - p0[2];
+        a[f][0] = y2 * z3 - z2 * y3;
+        a[f][1] = z2 * x3 - x2 * z3;
+        a[f][2] = x2 * y3 - y2 * x3;
     }
-    const float V = fabsf((E[3][0] * a0 + E[3][1] * b0 + E[3][2] * c0) * (1.0f / 6.0f));
+    const float V = fabsf((E[3][0] * a[0][0] + E[3][1] * a[0][1] + E[3][2] * a[0][2]) * (1.0f / 6.0f));
     o.V = V;
-    const float iV3 = 1.0f / (3.0f * V);
-#define TF(k, p, q) (U[k][p] * n[((k) + 1) & 3][q] * A[((k) + 1) & 3])
-    const float s00 = TF(0, 0, 0) + TF(1, 0, 0) + TF(2, 0, 0) + TF(3, 0, 0);
-    const float s11 = TF(0, 1, 1) + TF(1, 1, 1) + TF(2, 1, 1) + TF(3, 1, 1);
-    const float s22 = TF(0, 2, 2) + TF(1, 2, 2) + TF(2, 2, 2) + TF(3, 2, 2);
-    const float s01 = TF(0, 0, 1) + TF(1, 0, 1) + TF(2, 0, 1) + TF(3, 0, 1);
-    const float s02 = TF(0, 0, 2) + TF(1, 0, 2) + TF(2, 0, 2) + TF(3, 0, 2);
-    const float s12 = TF(0, 1, 2) + TF(1, 1, 2) + TF(2, 1, 2) + TF(3, 1, 2);
-    const float s10 = TF(0, 1, 0) + TF(1, 1, 0) + TF(2, 1, 0) + TF(3, 1, 0);
-    const float s20 = TF(0, 2, 0) + TF(1, 2, 0) + TF(2, 2, 0) + TF(3, 2, 0);
-    const float s21 = TF(0, 2, 1) + TF(1, 2, 1) + TF(2, 2, 1) + TF(3, 2, 1);
-#undef TF
-    o.sr[0] = -s00 * iV3;
-    o.sr[1] = -s11 * iV3;
-    o.sr[2] = -s22 * iV3;
-    o.sr[3] = -0.5f * (s01 + s10) * iV3;
-    o.sr[4] = -0.5f * (s02 + s20) * iV3;
-    o.sr[5] = -0.5f * (s12 + s21) * iV3;
-    o.w[0] = -0.5f * (s01 - s10) * iV3;
-    o.w[1] = -0.5f * (s02 - s20) * iV3;
-    o.w[2] = -0.5f * (s12 - s21) * iV3;
+    float G[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            G[p][q] = U[0][p] * a[1][q] + U[1][p] * a[2][q] + U[2][p] * a[3][q] + U[3][p] * a[0][q];
+    const float g = -__builtin_amdgcn_rcpf(6.0f * V);   /* L = g G */
+    o.sr[0] = g * G[0][0];
+    o.sr[1] = g * G[1][1];
+    o.sr[2] = g * G[2][2];
+    o.sr[3] = 0.5f * g * (G[0][1] + G[1][0]);
+    o.sr[4] = 0.5f * g * (G[0][2] + G[2][0]);
+    o.sr[5] = 0.5f * g * (G[1][2] + G[2][1]);
+    o.w[0] = 0.5f * g * (G[0][1] - G[1][0]);
+    o.w[1] = 0.5f * g * (G[0][2] - G[2][0]);
+    o.w[2] = 0.5f * g * (G[1][2] - G[2][1]);
 }
 
 /* Jaumann rotation as FP32 increments applied in the reference's order to
@@ -93,17 +81,20 @@ __device__ __forceinline__ void rotate_stress_mx(double st[6], const float w[3],
     st[5] += (double)(w12 * (float)(st[2] - st[1]) - w01 * (float)st[4] - w02 * (float)st[3]);
 }
 
-template <int RSET>
-__global__ void __launch_bounds__(64) k_elem_mx(StepArgs a) {
-    __shared__ double sh_a[EPW][NTET];
-    __shared__ double sh_b[EPW][NTET];
-    __shared__ double sh_sp[EPW][NTET][5];
-    __shared__ double sh_ps[EPW][NTET];
+/* TW as in k_elem: one wave per tet index over 64 elements */
+template <int RSET, bool TW>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
+    constexpr int NE = TW ? 64 : EPW;
+    constexpr int NT = TW ? NTET + 1 : NTET;
+    __shared__ double sh_a[NE][NT];
+    __shared__ double sh_b[NE][NT];
+    __shared__ double sh_sp[NE][NT][5];
+    __shared__ double sh_ps[NE][NT];
     const int lane = threadIdx.x;
-    const bool in_wave = lane < EPW * NTET;
-    const int le = in_wave ? lane / NTET : 0;
-    const int t = in_wave ? lane - le * NTET : 0;
-    const int e = xcd_block() * EPW + le;
+    const bool in_wave = TW || lane < EPW * NTET;
+    const int le = TW ? (lane & 63) : (in_wave ? lane / NTET : 0);
+    const int t = TW ? (lane >> 6) : (in_wave ? lane - le * NTET : 0);
+    const int e = xcd_block() * NE + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;

@@ -18,9 +18,11 @@ pytestmark = pytest.mark.gpu
 TOPO_TOL = 1e-6
 
 
-def _run(p, prec, nsteps, perturb=0.0):
+def _run(p, prec, nsteps, perturb=0.0, variant=None):
     p.precision = prec
     c = Context(p).Construct().Build().Initialise()
+    if variant is not None:
+        c.set_elem_variant(variant)
     x0 = c.state(fields=("coord",))["coord"]
     if perturb:
         st = c.state(fields=("stress",))["stress"]
@@ -54,13 +56,14 @@ def _report(tag, err):
     lambda: P.maxwell_viscoelastic(elements=(16, 6, 16)),
     lambda: P.elastic_box_compression(elements=(16, 8, 16)),
 ], ids=["maxwell", "elastic"])
-def test_mixed_undamped_topography_error(make):
+@pytest.mark.parametrize("variant", [_abi.ELEM_TET_PER_LANE, _abi.ELEM_TET_PER_WAVE], ids=["tet_per_lane", "tet_per_wave"])
+def test_mixed_undamped_topography_error(make, variant):
     p = make()
     p.demf = 0.0
     x0, a, sa = _run(p, _abi.PRECISION_FP64, 300)
-    _, b, sb = _run(p, _abi.PRECISION_MIXED, 300)
+    _, b, sb = _run(p, _abi.PRECISION_MIXED, 300, variant=variant)
     err = _errors(p, x0, a, sa, b, sb)
-    _report(f"{p.name} demf=0 mixed vs fp64, 300 steps", err)
+    _report(f"{p.name} demf=0 mixed ({variant}) vs fp64, 300 steps", err)
     assert err["topography"] <= TOPO_TOL
 
 
@@ -80,3 +83,24 @@ def test_mixed_damped_error_within_fp64_sensitivity(make):
     _report(f"{p.name} demf=0.8 fp64 with 1e-12 stress perturbation vs fp64", floor)
     assert np.isfinite(mixed["topography"])
     assert mixed["topography"] <= max(10.0 * floor["topography"], TOPO_TOL)
+
+
+def test_mixed_layouts_agree():
+    """The tet-per-lane and tet-per-wave layouts of the mixed element pass do
+    the same FP32 operations in the same order: identical results."""
+    p = P.mohr_coulomb_extension(elements=(16, 6, 8), seed=1)
+    _, a, _ = _run(p, _abi.PRECISION_MIXED, 50, variant=_abi.ELEM_TET_PER_LANE)
+    _, b, _ = _run(p, _abi.PRECISION_MIXED, 50, variant=_abi.ELEM_TET_PER_WAVE)
+    for f in ("coord", "velocity", "stress", "plastic_strain"):
+        assert np.array_equal(a[f], b[f]), f
+
+
+def test_mixed_rejects_diagnostics():
+    p = P.elastic_box_compression(elements=(4, 3, 4))
+    p.precision = _abi.PRECISION_MIXED
+    c = Context(p).Construct().Build().Initialise()
+    try:
+        with pytest.raises(Exception):
+            c.set_diagnostics(True)
+    finally:
+        c.Destroy()
