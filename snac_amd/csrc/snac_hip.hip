@@ -438,20 +438,35 @@ __device__ __forceinline__ bool mc_surely_elastic(const DevMaterial& m, const do
 /* eigen-decomposition of the symmetric trial stress and Mohr-Coulomb return
    for a tet that may yield; out of line, so that the registers it needs are
    not reserved across the whole element loop (most tets never call it).
-   Returns whether the tet yielded; st then holds the returned stress. */
-__device__ __noinline__ bool mc_plastic_update(const DevMaterial& m, double* st, double sphi, double spsi,
-                                               double coh, double tc, double* dps) {
+   io[0..5] holds the trial stress; on yield it holds the returned stress and
+   io[6] the plastic strain increment. */
+__device__ __noinline__ bool mc_plastic_update(const DevMaterial& m, double* io, double sphi, double spsi,
+                                               double coh, double tc) {
     double S[3][3];
-    S[0][0] = st[0]; S[1][1] = st[1]; S[2][2] = st[2];
-    S[0][1] = S[1][0] = st[3];
-    S[0][2] = S[2][0] = st[4];
-    S[1][2] = S[2][1] = st[5];
+    S[0][0] = io[0]; S[1][1] = io[1]; S[2][2] = io[2];
+    S[0][1] = S[1][0] = io[3];
+    S[0][2] = S[2][0] = io[4];
+    S[1][2] = S[2][1] = io[5];
     double d3[3], cn[3][3], dd = 0.0;
     principal_stresses(S, d3, cn);
     if (!mc_return(m, S, d3, cn, sphi, spsi, coh, tc, 0.0, false, dd)) return false;
-    st[0] = S[0][0]; st[1] = S[1][1]; st[2] = S[2][2];
-    st[3] = S[0][1]; st[4] = S[0][2]; st[5] = S[1][2];
-    *dps = dd;
+    io[0] = S[0][0]; io[1] = S[1][1]; io[2] = S[2][2];
+    io[3] = S[0][1]; io[4] = S[0][2]; io[5] = S[1][2];
+    io[6] = dd;
+    return true;
+}
+
+/* The call passes a local copy, made only on the path that calls, so the
+   caller's st stays in registers instead of living in scratch memory for
+   the whole kernel (an array whose address reaches an out-of-line call
+   cannot be promoted). */
+__device__ __forceinline__ bool mc_plastic_step(const DevMaterial& m, double st[6], double sphi, double spsi,
+                                                double coh, double tc, double& dps) {
+    double io[7] = {st[0], st[1], st[2], st[3], st[4], st[5], 0.0};
+    if (!mc_plastic_update(m, io, sphi, spsi, coh, tc)) return false;
+#pragma unroll
+    for (int c = 0; c < 6; c++) st[c] = io[c];
+    dps = io[6];
     return true;
 }
 
@@ -701,7 +716,7 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
         double sphi, spsi, dps = 0.0;
         angle_terms(m, sp.phi, sp.psi, sphi, spsi);
         bool changed = false;
-        if (!mc_surely_elastic(m, st, sphi, coh, tc) && mc_plastic_update(m, st, sphi, spsi, coh, tc, &dps)) {
+        if (!mc_surely_elastic(m, st, sphi, coh, tc) && mc_plastic_step(m, st, sphi, spsi, coh, tc, dps)) {
             ps = ps + dps;
             changed = true;
         }
@@ -927,7 +942,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             double sphi, spsi;
             angle_terms(m, pphi, ppsi, sphi, spsi);
             double dps = 0.0;
-            if (!mc_surely_elastic(m, st, sphi, coh, ptc) && mc_plastic_update(m, st, sphi, spsi, coh, ptc, &dps))
+            if (!mc_surely_elastic(m, st, sphi, coh, ptc) && mc_plastic_step(m, st, sphi, spsi, coh, ptc, dps))
                 plasticStrain = plasticStrain + dps;
             /* in place, an unchanged value need not be stored */
             if (a.ps_out != a.ps || plasticStrain != a.ps[(size_t)t * E + e])

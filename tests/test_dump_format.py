@@ -4,13 +4,55 @@ record sizes, float32 and int32 encodings, the dump cadence of the reference
 import numpy as np
 
 from snac_amd import problem as P
-from snac_amd.dump import DumpWriter, is_time_to_dump, stress_tensor_float
+from snac_amd.dump import (DumpWriter, is_time_to_checkpoint, is_time_to_dump, log10_mean_viscosity, read_checkpoint,
+                           stress_tensor_float)
 from tests.oracle_context import OracleContext
 
 
 def test_cadence():
     assert [t for t in range(1, 23) if is_time_to_dump(t, 5)] == [1, 5, 10, 15, 20]
     assert not is_time_to_dump(0, 5)
+
+
+def test_checkpoint_cadence():
+    # isTimeToCheckpoint (Context.c:1545-1561): every checkpointEvery steps
+    # and at maxTimeSteps, never at step 0, never with checkpointEvery = 0
+    assert [t for t in range(0, 23) if is_time_to_checkpoint(t, 10, 22)] == [10, 20, 22]
+    assert [t for t in range(0, 23) if is_time_to_checkpoint(t, 0, 22)] == [22]
+    assert [t for t in range(0, 23) if is_time_to_checkpoint(t, 0, None)] == []
+
+
+def test_log10_viscosity_matches_float_loop():
+    rng = np.random.default_rng(3)
+    v = np.exp(rng.normal(np.log(1e21), 1.0, size=(7, 10)))
+    got = log10_mean_viscosity(v)
+    for e in range(7):
+        acc = np.float32(0.0)
+        for t in range(10):
+            acc = np.float32(float(acc) + v[e, t] / 10)
+        assert got[e] == np.float32(np.log10(float(acc)))
+
+
+def test_checkpoint_files_and_reader(tmp_path):
+    p = P.maxwell_viscoelastic(elements=(4, 3, 4))
+    o = OracleContext(p)
+    w = DumpWriter(str(tmp_path), o, dump_every=100, checkpoint_every=3, max_time_steps=7)
+    w.step(7)          # checkpoints after steps 3, 6 and 7
+    w.close()
+    E, N = p.n_elements, p.n_nodes
+    sizes = {"coordCP": 12 * N, "velCP": 12 * N, "phaseIndexCP": 4 * E, "stressTensorCP": 240 * E,
+             "viscosityCP": 4 * E}
+    for name, rec in sizes.items():
+        assert (tmp_path / f"{name}.0").stat().st_size == 3 * rec, name
+    assert not (tmp_path / "plStrainCP.0").exists()
+    recs = read_checkpoint(str(tmp_path), p)
+    assert [r["time_step"] for r in recs] == [3, 6, 7]
+    st = o.state()
+    assert np.array_equal(recs[-1]["coord"], st["coord"].astype(np.float32))
+    assert np.array_equal(recs[-1]["velocity"], st["velocity"].astype(np.float32))
+    assert np.array_equal(recs[-1]["stress"], st["stress"].astype(np.float32))
+    assert np.array_equal(recs[-1]["log10_viscosity"], log10_mean_viscosity(st["viscosity"]))
+    assert recs[-1]["dt"] > 0 and recs[0]["time"] < recs[-1]["time"]
 
 
 def test_files_from_oracle_state(tmp_path):

@@ -1,5 +1,6 @@
-"""Dump diagnostics and the reference's output files from a GPU run, against
-the same files written from the CPU oracle's state by the same writer."""
+"""Dump diagnostics and the reference's output and checkpoint files from a
+GPU run, against the same files written from the CPU oracle's state by the
+same writer."""
 import os
 
 import numpy as np
@@ -42,12 +43,12 @@ def test_diagnostics_match_oracle(case):
 def test_dump_files_match_oracle(case, tmp_path):
     p = CASES[case]()
     c = Context(p).Construct().Build().Initialise()
-    wg = DumpWriter(str(tmp_path / "gpu"), c, dump_every=5)
+    wg = DumpWriter(str(tmp_path / "gpu"), c, dump_every=5, checkpoint_every=4, max_time_steps=12)
     wg.step(12)
     wg.close()
     c.Destroy()
     o = OracleContext(p)
-    wo = DumpWriter(str(tmp_path / "cpu"), o, dump_every=5)
+    wo = DumpWriter(str(tmp_path / "cpu"), o, dump_every=5, checkpoint_every=4, max_time_steps=12)
     wo.step(12)
     wo.close()
     names = sorted(os.listdir(tmp_path / "cpu"))
@@ -57,6 +58,8 @@ def test_dump_files_match_oracle(case, tmp_path):
         b = (tmp_path / "gpu" / name).read_bytes()
         print(f"  {case} {name}: {len(a)} bytes, identical {a == b}", flush=True)
         assert a == b, name
-    # steps 1, 5 and 10 are dumped: three records per file
+    # steps 1, 5 and 10 are dumped: three records per file; steps 4, 8
+    # and 12 are checkpointed
     E = p.n_elements
     assert (tmp_path / "gpu" / "stress.0").stat().st_size == 3 * 4 * E
+    assert (tmp_path / "gpu" / "stressTensorCP.0").stat().st_size == 3 * 240 * E
