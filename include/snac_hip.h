@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SNH_ABI_VERSION 1
+#define SNH_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------- */
 #define SNH_OK            0
@@ -97,6 +97,8 @@ typedef struct snh_material {
     double   ten_off;             /* tension cut-off; 0 => cohesion/tan(phi)  */
     double   vis_min, vis_max, refvisc, refsrate, reftemp;
     double   activationE, srexponent;
+    double   thermal_conduct;     /* W/m/K, read by the energy entry points    */
+    double   heatCapacity;        /* J/kg/K                                     */
 } snh_material;
 
 /* One velocity condition of the velocityBCs composite (StGermain/Base/Automation/
@@ -135,6 +137,8 @@ typedef struct snh_config {
  *   plastic_strain [E][10]       SnacPlastic_Element.plasticStrain
  *   viscosity      [E][10]       SnacMaxwell_Element.viscosity (download only)
  *   volume         [E][10]       tetra volume of the current geometry (download only)
+ *   temperature    [N]           SnacTemperature_Node.temperature (contexts
+ *                                with snh_set_thermal)
  * Node index n = i + I*(j + J*k); element index e = i + (I-1)*(j + (J-1)*k)
  * (StGermain/Discretisation/Mesh/src/HexaEL.c:400-426). */
 typedef struct snh_state {
@@ -144,6 +148,7 @@ typedef struct snh_state {
     double* plastic_strain;
     double* viscosity;
     double* volume;
+    double* temperature;
 } snh_state;
 
 /* Step scalars kept by the context (Snac_Context fields). */
@@ -191,6 +196,38 @@ int snh_download_state(snh_context* ctx, snh_state* state);
  * computation on the current coordinates, then uploaded. */
 int snh_hydrostatic_ic(snh_context* ctx);
 
+/* One condition of the temperatureBCs composite (a composite of wall
+ * conditions, like velocityBCs; later entries win on shared nodes). */
+typedef struct snh_temperature_bc {
+    int32_t  wall;                /* SNH_WALL_*                               */
+    double   value;               /* temperature on that wall                  */
+} snh_temperature_bc;
+
+/* The SnacTemperature plugin (Snac/plugins/temperature/Register.c:77-140):
+ * the energy entry points run at the start of every step
+ * (_Snac_Context_Solve, Context.c:1073-1074):
+ *   Snac_EP_LoopElementsEnergy -> SnacTemperature_LoopElements
+ *       per-tet heat flux k grad T from the current geometry, the heat-flux
+ *       condition on the vertical walls, thermal stress
+ *       (HeatFlux.c:47-155, VariableConditions.c:227-353)
+ *   Snac_EP_LoopNodesEnergy -> SnacTemperature_LoopNodes
+ *       nodal temperature update, tet average temperature, temperature BCs
+ *       (Heat.c:47-172)
+ * The nodal temperature then enters the Arrhenius viscosity of the Maxwell
+ * and visco-plastic laws, the tet density (alpha) and, with
+ * computeThermalStress, the stress of those laws.  Call after create and
+ * before snh_timestep_zero; temperature[N] is the initial nodal field (the
+ * nodeICs), to which _SnacTemperature_InitialConditions (VariableConditions.c:
+ * 155-212) applies the BCs.  FP64 contexts only (SNH_EUNSUPPORTED for
+ * mixed precision and the adjoint). */
+typedef struct snh_thermal {
+    int32_t  computeThermalStress;   /* Context.c:756-757                   */
+    uint32_t nbcs;
+    const snh_temperature_bc* bcs;
+    const double* temperature;       /* [N] initial nodal temperature         */
+} snh_thermal;
+int snh_set_thermal(snh_context* ctx, const snh_thermal* thermal);
+
 /* Snac_Context_TimeStepZero (Snac/libSnac/src/Context.c:978-1036): geometry of
  * the initial mesh, minLengthScale and the first Sync (dt). */
 int snh_timestep_zero(snh_context* ctx);
@@ -205,6 +242,8 @@ int snh_timestep_zero(snh_context* ctx);
  * The Jaumann rotation computed by LoopElementsMomentum is applied to the
  * stored stress at the start of the next CalcStresses (same values, same
  * order); snh_download_state returns the rotated stress in between. */
+int snh_loop_elements_energy(snh_context* ctx);   /* Snac_EP_LoopElementsEnergy (Context.c:1073) */
+int snh_loop_nodes_energy(snh_context* ctx);      /* Snac_EP_LoopNodesEnergy (Context.c:1074) */
 int snh_calc_stresses(snh_context* ctx);
 int snh_loop_nodes_momentum(snh_context* ctx);
 int snh_loop_elements_momentum(snh_context* ctx);
@@ -227,6 +266,8 @@ int snh_synchronize(snh_context* ctx);
 typedef struct snh_kernel_times {
     double   elem_ms, node_ms, geo_ms, fin_ms;
     uint32_t elem_n, node_n, geo_n, fin_n;
+    double   heat_elem_ms, heat_node_ms;     /* energy entry points */
+    uint32_t heat_elem_n, heat_node_n;
 } snh_kernel_times;
 int snh_profile(snh_context* ctx, int enable);
 

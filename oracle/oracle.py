@@ -91,6 +91,10 @@ def lib():
             "orc_set_glibc_hypot": (None, [I]),
             "orc_record_decisions": (None, [P, C.POINTER(C.c_int8), C.POINTER(C.c_int8)]),
             "orc_get_bc": (I, [P, C.POINTER(C.c_uint8), DP]),
+            "orc_set_thermal": (I, [P, C.POINTER(_abi.Thermal)]),
+            "orc_loop_elements_energy": (I, [P]),
+            "orc_loop_nodes_energy": (I, [P]),
+            "orc_get_heat": (I, [P, U, U, DP, DP, DP, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -134,6 +138,10 @@ class Oracle:
             st = _abi.State()
             st.plastic_strain = _dp(ps)
             _check(L.orc_set_state(h, C.byref(st)))
+        self.thermal = problem.thermal is not None
+        if self.thermal:
+            th, keep = problem.thermal.to_c()
+            _check(L.orc_set_thermal(h, C.byref(th)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -188,12 +196,16 @@ class Oracle:
         ps = np.zeros((self.E, 10))
         visc = np.zeros((self.E, 10))
         vol = np.zeros((self.E, 10))
-        st = _abi.State(_dp(coord), _dp(vel), _dp(stress), _dp(ps), _dp(visc), _dp(vol))
+        temp = np.zeros(self.N)
+        st = _abi.State(_dp(coord), _dp(vel), _dp(stress), _dp(ps), _dp(visc), _dp(vol), _dp(temp))
         _check(lib().orc_get_state(self.h, C.byref(st)))
-        return {"coord": coord, "velocity": vel, "stress": stress, "plastic_strain": ps,
-                "viscosity": visc, "volume": vol}
+        out = {"coord": coord, "velocity": vel, "stress": stress, "plastic_strain": ps,
+               "viscosity": visc, "volume": vol}
+        if self.thermal:
+            out["temperature"] = temp
+        return out
 
-    def set_state(self, coord=None, velocity=None, stress=None, plastic_strain=None):
+    def set_state(self, coord=None, velocity=None, stress=None, plastic_strain=None, temperature=None):
         keep = []
 
         def conv(a, shape):
@@ -203,7 +215,8 @@ class Oracle:
             keep.append(a)
             return _dp(a)
         st = _abi.State(conv(coord, (self.N, 3)), conv(velocity, (self.N, 3)),
-                        conv(stress, (self.E, 10, 6)), conv(plastic_strain, (self.E, 10)), None, None)
+                        conv(stress, (self.E, 10, 6)), conv(plastic_strain, (self.E, 10)), None, None,
+                        conv(temperature, (self.N,)))
         _check(lib().orc_set_state(self.h, C.byref(st)))
 
     # granular functions -------------------------------------------------
@@ -244,6 +257,18 @@ class Oracle:
 
     def sync(self):
         _check(lib().orc_sync(self.h))
+
+    def loop_elements_energy(self):
+        _check(lib().orc_loop_elements_energy(self.h))
+
+    def loop_nodes_energy(self):
+        _check(lib().orc_loop_nodes_energy(self.h))
+
+    def heat(self, e, t):
+        hf = np.zeros(3)
+        ts, at, rho = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().orc_get_heat(self.h, e, t, _dp(hf), C.byref(ts), C.byref(at), C.byref(rho)))
+        return {"heatFlux": hf, "thermalStress": ts.value, "avgTemp": at.value, "density": rho.value}
 
     def tet(self, e, t):
         v = TetView()

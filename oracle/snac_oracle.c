@@ -29,6 +29,10 @@
  *   maxwell                       constant-strain relaxation ((1-a)/(1+a))^n
  *   viscoplastic_BI               Maxwell limit below yield; diagonal-only
  *                                 eigen-solve on yield (lower triangle zero)
+ *   temperature (energy EPs)      uniform and linear fields (zero flux / exact
+ *                                 gradient, steady interior), insulated-box
+ *                                 energy conservation, 1D diffusion decay rate,
+ *                                 thermal stress closed form
  * hypot                         correctly rounded (binary128), checked against
  *                                 exact rational arithmetic; glibc's is selectable
  * libm (sin, tan, pow, exp) is the system's glibc: parity unpinned at that
@@ -114,7 +118,14 @@ struct orc_context {
     orc_element* elem;
     unsigned (*elementNodeTbl)[8];
     unsigned (*nodeElementTbl)[8];
-    double* temperature;           /* SnacTemperature_Node.temperature (zero) */
+    double* temperature;           /* SnacTemperature_Node.temperature (zero
+                                      unless the temperature plugin is on) */
+    double* temperature0;          /* SnacTemperature_Node.temperature0 */
+    double (*heatFlux)[NTET][3];   /* SnacTemperature_Element.heatFlux */
+    double (*thermalStress)[NTET]; /* SnacTemperature_Element.thermalStress */
+    int thermal, thermal_stress;
+    unsigned char* tbc_mask;       /* temperature BC per node */
+    double* tbc_value;
     double* refvisc_field;         /* per-element refvisc, or NULL */
     double* coh_offset_field;      /* per-element cohesion offset, or NULL */
     /* velocity BCs: per-node table and the index table of the composite */
@@ -340,6 +351,7 @@ void orc_destroy(orc_context* c) {
     free(c->coord); free(c->node); free(c->elem);
     free(c->elementNodeTbl); free(c->nodeElementTbl);
     free(c->temperature); free(c->refvisc_field); free(c->coh_offset_field);
+    free(c->temperature0); free(c->heatFlux); free(c->thermalStress); free(c->tbc_mask); free(c->tbc_value);
     free(c->bc_mask); free(c->bc_value); free(c->bc_index);
     free(c);
 }
@@ -389,6 +401,7 @@ int orc_set_node_temperature(orc_context* c, const double* t, size_t nnode) {
 
 int orc_set_state(orc_context* c, const snh_state* s) {
     if (s->coord) memcpy(c->coord, s->coord, sizeof(double) * 3 * c->N);
+    if (s->temperature) memcpy(c->temperature, s->temperature, sizeof(double) * c->N);
     if (s->velocity)
         for (unsigned n = 0; n < c->N; n++)
             for (int d = 0; d < 3; d++) c->node[n].velocity[d] = s->velocity[3 * n + d];
@@ -410,6 +423,7 @@ int orc_set_state(orc_context* c, const snh_state* s) {
 
 int orc_get_state(orc_context* c, snh_state* s) {
     if (s->coord) memcpy(s->coord, c->coord, sizeof(double) * 3 * c->N);
+    if (s->temperature) memcpy(s->temperature, c->temperature, sizeof(double) * c->N);
     if (s->velocity)
         for (unsigned n = 0; n < c->N; n++)
             for (int d = 0; d < 3; d++) s->velocity[3 * n + d] = c->node[n].velocity[d];
@@ -984,6 +998,11 @@ static int constitutive_maxwell(orc_context* c, unsigned e) {
         double(*st)[3] = el->tetra[t].stress;
         const double(*sn)[3] = el->tetra[t].strain;
         double rmu = m->mu;
+        if (c->thermal_stress) {   /* :106-110 */
+            st[0][0] += c->thermalStress[e][t];
+            st[1][1] += c->thermalStress[e][t];
+            st[2][2] += c->thermalStress[e][t];
+        }
         double trace_stress = st[0][0] + st[1][1] + st[2][2];
         double trace_strain = el->tetra[t].volume / el->tetra[t].old_volume - 1.0;
         double straind0 = sn[0][0] - (trace_strain) / 3.0;
@@ -1032,6 +1051,11 @@ static int constitutive_viscoplastic(orc_context* c, unsigned e) {
         double(*st)[3] = el->tetra[t].stress;
         const double(*sn)[3] = el->tetra[t].strain;
         double plasticStrain = el->plasticStrain[t];
+        if (c->thermal_stress) {   /* :145-149 */
+            st[0][0] += c->thermalStress[e][t];
+            st[1][1] += c->thermalStress[e][t];
+            st[2][2] += c->thermalStress[e][t];
+        }
         double trace_stress = st[0][0] + st[1][1] + st[2][2];
         double trace_strain = sn[0][0] + sn[1][1] + sn[2][2];
         double straind0 = sn[0][0] - (trace_strain) / 3.0;
@@ -1259,6 +1283,203 @@ int orc_update_node(orc_context* c, uint32_t n) {
 }
 
 /* ---------------------------------------------------------------------------
+ * SnacTemperature plugin: the energy entry points
+ * ------------------------------------------------------------------------- */
+static void apply_temperature_bc(orc_context* c) {
+    for (unsigned n = 0; n < c->N; n++)
+        if (c->tbc_mask[n]) c->temperature[n] = c->tbc_value[n];
+}
+
+/* _SnacTemperature_InitialConditions (temperature/VariableConditions.c:
+   155-212): T0 = T (no restart), temperature BCs, zero heat flux.  The
+   temperatureBCs composite of wall conditions: later entries win. */
+int orc_set_thermal(orc_context* c, const snh_thermal* th) {
+    if (!th || (th->nbcs && !th->bcs)) return fail("bad thermal description");
+    if (!c->temperature0) {
+        c->temperature0 = calloc(c->N, sizeof(double));
+        c->heatFlux = calloc(c->E, sizeof *c->heatFlux);
+        c->thermalStress = calloc(c->E, sizeof *c->thermalStress);
+        c->tbc_mask = calloc(c->N, 1);
+        c->tbc_value = calloc(c->N, sizeof(double));
+        if (!c->temperature0 || !c->heatFlux || !c->thermalStress || !c->tbc_mask || !c->tbc_value)
+            return fail("out of memory");
+    }
+    c->thermal = 1;
+    c->thermal_stress = th->computeThermalStress != 0;
+    if (th->temperature) memcpy(c->temperature, th->temperature, sizeof(double) * c->N);
+    for (unsigned n = 0; n < c->N; n++) {
+        c->tbc_mask[n] = 0;
+        for (unsigned b = 0; b < th->nbcs; b++)
+            if (node_on_wall(c, n, th->bcs[b].wall)) {
+                c->tbc_mask[n] = 1;
+                c->tbc_value[n] = th->bcs[b].value;
+            }
+    }
+    memcpy(c->temperature0, c->temperature, sizeof(double) * c->N);
+    apply_temperature_bc(c);
+    memset(c->heatFlux, 0, sizeof(*c->heatFlux) * c->E);
+    return SNH_OK;
+}
+
+/* Snac_HeatFlux (temperature/HeatFlux.c:63-155): per tet, k grad T from the
+   stored geometry (node k pairs with surface (k+1)%4), and the thermal stress
+   -alpha bulkm (mean T - mean T0). */
+static void heat_flux(orc_context* c, unsigned e) {
+    orc_element* el = &c->elem[e];
+    const snh_material* m = &c->mat[el->material_I];
+    const double bulkm = m->lambda + 2.0 * m->mu / 3.0;
+    const double alpha = m->alpha;
+    for (int t = 0; t < NTET; t++) {
+        const orc_tet* tt = &el->tetra[t];
+        const orc_surface* S = tt->surface;
+        double T[4], T0[4];
+        for (int k = 0; k < 4; k++) {
+            unsigned nd = c->elementNodeTbl[e][TetraToNode[t][k]];
+            T[k] = c->temperature[nd];
+            T0[k] = c->temperature0[nd];
+        }
+        for (int d = 0; d < 3; d++)
+            c->heatFlux[e][t][d] = 1.0 * m->thermal_conduct *
+                                   (T[0] * S[1].normal[d] * S[1].area + T[1] * S[2].normal[d] * S[2].area +
+                                    T[2] * S[3].normal[d] * S[3].area + T[3] * S[0].normal[d] * S[0].area) /
+                                   3.0 / tt->volume;
+        if (c->thermal_stress) {
+            double temp = 0.25 * (T[0] + T[1] + T[2] + T[3]);
+            double temp0 = 0.25 * (T0[0] + T0[1] + T0[2] + T0[3]);
+            c->thermalStress[e][t] = -1.0 * alpha * bulkm * (temp - temp0);
+        }
+    }
+}
+
+/* SnacTemperature_HeatFluxBC (temperature/VariableConditions.c:227-353): on
+   elements of the x and z walls, remove the normal component of -grad T
+   (without the conductivity) from the flux; the wall normal is the mean of
+   four tet surface normals, the last matching wall's. */
+static void heat_flux_bc(orc_context* c, unsigned e) {
+    static const unsigned wall_faces[4][4][2] = {
+        {{0, 2}, {1, 2}, {5, 0}, {6, 0}},   /* i == 0 */
+        {{2, 0}, {3, 0}, {7, 0}, {8, 0}},   /* i == EI-1 */
+        {{0, 1}, {2, 2}, {6, 1}, {7, 1}},   /* k == 0 */
+        {{1, 1}, {3, 2}, {5, 1}, {8, 1}}};  /* k == EK-1 */
+    const unsigned i = e % c->EI, k = e / (c->EI * c->EJ);
+    const int on[4] = {i == 0, i == c->EI - 1, k == 0, k == c->EK - 1};
+    if (!(on[0] || on[1] || on[2] || on[3])) return;
+    orc_element* el = &c->elem[e];
+    double normal[3];
+    for (int w = 0; w < 4; w++) {
+        if (!on[w]) continue;
+        for (int d = 0; d < 3; d++) {
+            const unsigned(*f)[2] = wall_faces[w];
+            normal[d] = 0.25 * (el->tetra[f[0][0]].surface[f[0][1]].normal[d] +
+                                el->tetra[f[1][0]].surface[f[1][1]].normal[d] +
+                                el->tetra[f[2][0]].surface[f[2][1]].normal[d] +
+                                el->tetra[f[3][0]].surface[f[3][1]].normal[d]);
+        }
+    }
+    for (int t = 0; t < NTET; t++) {
+        const orc_tet* tt = &el->tetra[t];
+        const orc_surface* S = tt->surface;
+        double T[4], g[3], Fn;
+        for (int kk = 0; kk < 4; kk++) T[kk] = c->temperature[c->elementNodeTbl[e][TetraToNode[t][kk]]];
+        for (int d = 0; d < 3; d++)
+            g[d] = -1.0 *
+                   (T[0] * S[1].normal[d] * S[1].area + T[1] * S[2].normal[d] * S[2].area +
+                    T[2] * S[3].normal[d] * S[3].area + T[3] * S[0].normal[d] * S[0].area) /
+                   3.0 / tt->volume;
+        Fn = 0.0;
+        for (int d = 0; d < 3; d++) Fn += g[d] * normal[d];
+        for (int d = 0; d < 3; d++) c->heatFlux[e][t][d] -= Fn * normal[d];
+    }
+}
+
+/* SnacTemperature_LoopElements (temperature/HeatFlux.c:47-61) */
+int orc_loop_elements_energy(orc_context* c) {
+    if (!c->thermal) return fail("no temperature plugin (orc_set_thermal)");
+    const long E = (long)c->E;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (long e = 0; e < E; e++) heat_flux(c, (unsigned)e);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (long e = 0; e < E; e++) heat_flux_bc(c, (unsigned)e);
+    return SNH_OK;
+}
+
+/* Snac_Heat (temperature/Heat.c:64-132) for node n with sourceterm 0.
+   Returns 0, or -1 when dt_thermal/dt_mech <= 1 (the firewall at :121-126). */
+static int node_heat(orc_context* c, unsigned n) {
+    double energy = 0.0, source = 0.0, lumpVolume = 0.0;
+    const double sourceterm = 0.0;
+    int bad = 0;
+    for (unsigned ne = 0; ne < 8; ne++) {
+        unsigned e = c->nodeElementTbl[n][ne];
+        if (e >= c->E) continue;
+        const orc_element* el = &c->elem[e];
+        const snh_material* m = &c->mat[el->material_I];
+        for (int idx = 0; idx < 5; idx++) {
+            const unsigned t = NodeToTetra[ne][idx];
+            const unsigned s = NodeToSurface[ne][idx];
+            const orc_tet* tt = &el->tetra[t];
+            const double* hf = c->heatFlux[e][t];
+            const orc_surface* sf = &tt->surface[s];
+            energy += (hf[0] * sf->normal[0] + hf[1] * sf->normal[1] + hf[2] * sf->normal[2]) * sf->area / 6.0;
+            source += sourceterm * tt->volume / 4.0;
+            lumpVolume +=
+                m->heatCapacity *
+                (((tt->density == 0.0)
+                      ? (m->phsDensity * (1.0 - m->alpha * (tt->avgTemp - m->reftemp) + m->beta * el->hydroPressure))
+                      : tt->density) *
+                 tt->volume) /
+                8.0;
+        }
+        double dt_thermal_to_mech = sqrt((m->lambda + 2.0 * m->mu) / m->phsDensity) * c->minLengthScale /
+                                    (m->thermal_conduct / (m->heatCapacity * m->phsDensity));
+        if (!(dt_thermal_to_mech > 1.0)) bad = 1;
+    }
+    c->temperature0[n] = c->temperature[n];
+    c->temperature[n] += (energy + source) * -1.0 * c->dt / lumpVolume;
+    return bad ? -1 : 0;
+}
+
+/* SnacTemperature_LoopNodes (temperature/Heat.c:47-62): nodal update, tet
+   average temperature (UpdateAverageTemp_LoopElements, :135-160) before the
+   BCs are re-applied, then the BCs. */
+int orc_loop_nodes_energy(orc_context* c) {
+    if (!c->thermal) return fail("no temperature plugin (orc_set_thermal)");
+    const long N = (long)c->N, E = (long)c->E;
+    int bad = 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static) reduction(min : bad)
+    for (long n = 0; n < N; n++) {
+        int r = node_heat(c, (unsigned)n);
+        if (r < bad) bad = r;
+    }
+    if (bad) {
+        snprintf(g_err, sizeof g_err, "timeStep=%u: dt_thermal / dt_mech <= 1 (Heat.c:121-126)", c->timeStep);
+        return SNH_ENUMERIC;
+    }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (long e = 0; e < E; e++) {
+        orc_element* el = &c->elem[e];
+        for (int t = 0; t < NTET; t++) {
+            el->tetra[t].avgTemp = 0.0;
+            for (int k = 0; k < 4; k++)
+                el->tetra[t].avgTemp += c->temperature[c->elementNodeTbl[e][TetraToNode[t][k]]] / (1.0 * 4);
+        }
+    }
+    apply_temperature_bc(c);
+    return SNH_OK;
+}
+
+int orc_get_heat(orc_context* c, uint32_t e, uint32_t t, double heatFlux[3], double* thermalStress,
+                 double* avgTemp, double* density) {
+    if (!c->thermal) return fail("no temperature plugin (orc_set_thermal)");
+    if (e >= c->E || t >= NTET) return fail("index out of range");
+    for (int d = 0; d < 3; d++) heatFlux[d] = c->heatFlux[e][t][d];
+    *thermalStress = c->thermalStress[e][t];
+    *avgTemp = c->elem[e].tetra[t].avgTemp;
+    *density = c->elem[e].tetra[t].density;
+    return SNH_OK;
+}
+
+/* ---------------------------------------------------------------------------
  * loop drivers (Context.c:1102-1184) and Sync (:1208-1268)
  * ------------------------------------------------------------------------- */
 int orc_calc_stresses(orc_context* c) {
@@ -1404,7 +1625,13 @@ int orc_step(orc_context* c, int nsteps) {
     }
     for (int s = 0; s < nsteps; s++) {
         c->timeStep++;
-        int rc = orc_calc_stresses(c);
+        int rc;
+        if (c->thermal) {   /* _Snac_Context_Solve, Context.c:1073-1074 */
+            orc_loop_elements_energy(c);
+            rc = orc_loop_nodes_energy(c);
+            if (rc) return rc;
+        }
+        rc = orc_calc_stresses(c);
         if (rc) return rc;
         orc_loop_nodes(c);
         orc_loop_elements(c);

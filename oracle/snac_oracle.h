@@ -63,6 +63,11 @@ int orc_get_bc(orc_context* c, uint8_t* mask, double* value);
 int orc_set_element_material(orc_context* c, const uint8_t* mat, size_t nelem);
 int orc_set_element_field(orc_context* c, int field, const double* v, size_t nelem);
 int orc_set_node_temperature(orc_context* c, const double* t, size_t nnode);
+int orc_set_thermal(orc_context* c, const snh_thermal* th);            /* temperature/VariableConditions.c:155-212 */
+int orc_loop_elements_energy(orc_context* c);                          /* temperature/HeatFlux.c:47-155 */
+int orc_loop_nodes_energy(orc_context* c);                             /* temperature/Heat.c:47-172 */
+int orc_get_heat(orc_context* c, uint32_t e, uint32_t t, double heatFlux[3], double* thermalStress,
+                 double* avgTemp, double* density);
 int orc_set_state(orc_context* c, const snh_state* s);
 int orc_get_state(orc_context* c, snh_state* s);
 int orc_get_scalars(orc_context* c, snh_scalars* out);

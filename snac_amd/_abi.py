@@ -62,6 +62,8 @@ class Material(C.Structure):
         ("reftemp", C.c_double),
         ("activationE", C.c_double),
         ("srexponent", C.c_double),
+        ("thermal_conduct", C.c_double),
+        ("heatCapacity", C.c_double),
     ]
 
 
@@ -102,6 +104,20 @@ class State(C.Structure):
         ("plastic_strain", C.POINTER(C.c_double)),
         ("viscosity", C.POINTER(C.c_double)),
         ("volume", C.POINTER(C.c_double)),
+        ("temperature", C.POINTER(C.c_double)),
+    ]
+
+
+class TemperatureBC(C.Structure):
+    _fields_ = [("wall", C.c_int32), ("value", C.c_double)]
+
+
+class Thermal(C.Structure):
+    _fields_ = [
+        ("computeThermalStress", C.c_int32),
+        ("nbcs", C.c_uint32),
+        ("bcs", C.POINTER(TemperatureBC)),
+        ("temperature", C.POINTER(C.c_double)),
     ]
 
 
@@ -117,7 +133,9 @@ class Scalars(C.Structure):
 
 class KernelTimes(C.Structure):
     _fields_ = [("elem_ms", C.c_double), ("node_ms", C.c_double), ("geo_ms", C.c_double), ("fin_ms", C.c_double),
-                ("elem_n", C.c_uint32), ("node_n", C.c_uint32), ("geo_n", C.c_uint32), ("fin_n", C.c_uint32)]
+                ("elem_n", C.c_uint32), ("node_n", C.c_uint32), ("geo_n", C.c_uint32), ("fin_n", C.c_uint32),
+                ("heat_elem_ms", C.c_double), ("heat_node_ms", C.c_double),
+                ("heat_elem_n", C.c_uint32), ("heat_node_n", C.c_uint32)]
 
 
 ELEM_TET_PER_LANE = 0
@@ -168,6 +186,9 @@ SIGNATURES = {
     "snh_download_state": (_I, [_P, C.POINTER(State)]),
     "snh_hydrostatic_ic": (_I, [_P]),
     "snh_timestep_zero": (_I, [_P]),
+    "snh_set_thermal": (_I, [_P, C.POINTER(Thermal)]),
+    "snh_loop_elements_energy": (_I, [_P]),
+    "snh_loop_nodes_energy": (_I, [_P]),
     "snh_calc_stresses": (_I, [_P]),
     "snh_loop_nodes_momentum": (_I, [_P]),
     "snh_loop_elements_momentum": (_I, [_P]),
@@ -195,6 +216,8 @@ SIGNATURES = {
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 
+ABI_VERSION = 2          # SNH_ABI_VERSION of include/snac_hip.h
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libsnac_hip.so")
 
@@ -212,6 +235,10 @@ def load_library(path=LIB_PATH):
             f"libsnac_hip.so not found at {path}; build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'`")
     lib = C.CDLL(path)
+    lib.snh_abi_version.restype = C.c_int
+    if lib.snh_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{path} has ABI version {lib.snh_abi_version()}, this binding expects {ABI_VERSION}; "
+                           "rebuild it with `python -c 'import __graft_entry__ as g; g.build()'`")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

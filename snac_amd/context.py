@@ -52,8 +52,12 @@ class Context:
         return self
 
     def Initialise(self):
-        """Initial conditions (plastic-strain seeds) then TimeStepZero."""
+        """Initial conditions (temperature plugin, hydrostatic stress,
+        plastic-strain seeds) then TimeStepZero."""
         p = self.problem
+        if p.thermal is not None:
+            th, keep = p.thermal.to_c()
+            self._check(self.lib.snh_set_thermal(self.h, C.byref(th)))
         if p.hydrostatic:
             self._check(self.lib.snh_hydrostatic_ic(self.h))
         if p.plastic_strain0 is not None:
@@ -89,6 +93,12 @@ class Context:
     # -- stepping ------------------------------------------------------------
     def step(self, nsteps: int = 1):
         self._check(self.lib.snh_step(self.h, int(nsteps)))
+
+    def loop_elements_energy(self):
+        self._check(self.lib.snh_loop_elements_energy(self.h))
+
+    def loop_nodes_energy(self):
+        self._check(self.lib.snh_loop_nodes_energy(self.h))
 
     def calc_stresses(self):
         self._check(self.lib.snh_calc_stresses(self.h))
@@ -193,7 +203,9 @@ class Context:
         k = _abi.KernelTimes()
         self._check(self.lib.snh_get_kernel_times(self.h, C.byref(k)))
         return {"elem_ms": k.elem_ms, "node_ms": k.node_ms, "geo_ms": k.geo_ms, "fin_ms": k.fin_ms,
-                "elem_n": k.elem_n, "node_n": k.node_n, "geo_n": k.geo_n, "fin_n": k.fin_n}
+                "elem_n": k.elem_n, "node_n": k.node_n, "geo_n": k.geo_n, "fin_n": k.fin_n,
+                "heat_elem_ms": k.heat_elem_ms, "heat_node_ms": k.heat_node_ms,
+                "heat_elem_n": k.heat_elem_n, "heat_node_n": k.heat_node_n}
 
     # -- state ----------------------------------------------------------------
     def scalars(self) -> Dict[str, float]:
@@ -202,9 +214,16 @@ class Context:
         return {"dt": s.dt, "speedOfSound": s.speedOfSound, "minLengthScale": s.minLengthScale,
                 "vmax": s.vmax, "timeStep": s.timeStep}
 
-    def state(self, fields=("coord", "velocity", "stress", "plastic_strain", "viscosity")) -> Dict[str, np.ndarray]:
+    def state(self, fields=None) -> Dict[str, np.ndarray]:
+        """Download fields (default: coord, velocity, stress, plastic_strain,
+        viscosity and, with the temperature plugin, temperature)."""
+        if fields is None:
+            fields = ("coord", "velocity", "stress", "plastic_strain", "viscosity")
+            if self.problem.thermal is not None:
+                fields = fields + ("temperature",)
         shapes = {"coord": (self.N, 3), "velocity": (self.N, 3), "stress": (self.E, 10, 6),
-                  "plastic_strain": (self.E, 10), "viscosity": (self.E, 10), "volume": (self.E, 10)}
+                  "plastic_strain": (self.E, 10), "viscosity": (self.E, 10), "volume": (self.E, 10),
+                  "temperature": (self.N,)}
         out = {f: np.zeros(shapes[f]) for f in fields}
         st = _abi.State()
         for f, a in out.items():
@@ -212,11 +231,12 @@ class Context:
         self._check(self.lib.snh_download_state(self.h, C.byref(st)))
         return out
 
-    def set_state(self, coord=None, velocity=None, stress=None, plastic_strain=None):
+    def set_state(self, coord=None, velocity=None, stress=None, plastic_strain=None, temperature=None):
         keep = []
         st = _abi.State()
         for name, a, shape in (("coord", coord, (self.N, 3)), ("velocity", velocity, (self.N, 3)),
-                               ("stress", stress, (self.E, 10, 6)), ("plastic_strain", plastic_strain, (self.E, 10))):
+                               ("stress", stress, (self.E, 10, 6)), ("plastic_strain", plastic_strain, (self.E, 10)),
+                               ("temperature", temperature, (self.N,))):
             if a is None:
                 continue
             a = np.ascontiguousarray(a, dtype=np.float64).reshape(shape)

@@ -470,10 +470,10 @@ __device__ __forceinline__ bool mc_plastic_step(const DevMaterial& m, double st[
     return true;
 }
 
-/* viscosity of maxwell (:123-151) / viscoplastic_BI (:174-218); the
-   temperature field is zero (SnacTemperature not on the path). */
+/* viscosity of maxwell (:123-151) / viscoplastic_BI (:174-218) at the tet's
+   mean nodal temperature avgTemp (zero without the temperature plugin) */
 __device__ __forceinline__ double viscosity_of(const DevMaterial& m, double rvisc, double sd0, double sd1, double sd2,
-                                               const double sn[6], double dt) {
+                                               const double sn[6], double dt, double avgTemp = 0.0) {
     double visc;
     if (m.linear_visc) {
         visc = rvisc; /* pow(x, 0) * exp(0) == 1 exactly */
@@ -481,13 +481,45 @@ __device__ __forceinline__ double viscosity_of(const DevMaterial& m, double rvis
         const double R = 8.31448;
         double srJ2 = sqrt(fabs(sd1 * sd2 + sd2 * sd0 + sd0 * sd1 - sn[3] * sn[3] - sn[4] * sn[4] - sn[5] * sn[5])) / dt;
         if (srJ2 == 0.0) srJ2 = m.refsrate;
-        double avgTemp = 0.0;
         visc = rvisc * pow((srJ2 / m.refsrate), (1. / m.srexponent - 1.)) *
                exp(m.activationE / R * (1. / (avgTemp + 273.15) - 1. / (m.reftemp + 273.15)));
     }
     if (visc < m.vis_min) visc = m.vis_min;
     if (visc > m.vis_max) visc = m.vis_max;
     return visc;
+}
+
+/* temperature coupling of one tet with nodes nd[] (temperature plugin on):
+   the mean nodal temperature of the viscous laws, 0.25 T summed in node
+   order (maxwell/Constitutive.c:124-132, viscoplastic_BI :178-186), only
+   needed by a non-linear law */
+__device__ __forceinline__ double tet_temp(const StepArgs& a, const DevMaterial& m, const int nd[4]) {
+    if (!a.temp || m.linear_visc) return 0.0;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) s += 0.25 * a.temp[nd[q]];
+    return s;
+}
+
+/* thermal stress on the diagonal before the viscous update
+   (maxwell/Constitutive.c:106-110, viscoplastic_BI :145-149) */
+__device__ __forceinline__ void add_thermal_stress(const StepArgs& a, double st[6], int t, size_t e) {
+    if (!a.tstress) return;
+    const double ts = a.tstress[(size_t)t * (size_t)a.E + e];
+    st[0] += ts;
+    st[1] += ts;
+    st[2] += ts;
+}
+
+/* the tet density set at the end of Stress (Stress.c:169-170) with beta = 0,
+   from the tet average of the temperatures computed before the BCs
+   (UpdateAverageTemp_LoopElements, temperature/Heat.c:135-160) */
+__device__ __forceinline__ void store_density(const StepArgs& a, const DevMaterial& m, const int nd[4], int t,
+                                              size_t e) {
+    double avg = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) avg += a.tpre[nd[q]] / 4.0;
+    a.dens[(size_t)t * (size_t)a.E + e] = m.phsDensity * (1.0 - m.alpha * (avg - m.reftemp));
 }
 
 /* deviatoric Maxwell update shared by maxwell and viscoplastic_BI */
@@ -561,6 +593,23 @@ __host__ __device__ __forceinline__ int tet_node(int t, int q) {
 __device__ __forceinline__ int hex_node_index(const StepArgs& a, int ei, int ej, int ek, int q) {
     const int di = (q ^ (q >> 1)) & 1, dj = (q >> 1) & 1, dk = q >> 2;
     return (ei + di) + a.I * ((ej + dj) + a.J * (ek + dk));
+}
+
+/* tet_temp / store_density for tet t of element (ei, ej, ek) */
+__device__ __forceinline__ double tet_temp_at(const StepArgs& a, const DevMaterial& m, int ei, int ej, int ek, int t) {
+    if (!a.temp || m.linear_visc) return 0.0;
+    int nd[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) nd[q] = hex_node_index(a, ei, ej, ek, kTetraToNode[t][q]);
+    return tet_temp(a, m, nd);
+}
+
+__device__ __forceinline__ void store_density_at(const StepArgs& a, const DevMaterial& m, int ei, int ej, int ek,
+                                                 int t, size_t e) {
+    int nd[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) nd[q] = hex_node_index(a, ei, ej, ek, kTetraToNode[t][q]);
+    store_density(a, m, nd, t, e);
 }
 
 /* softened parameters carried along the tets of an element: the reference
@@ -723,11 +772,12 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
         /* in place, an unchanged value need not be stored */
         if (active && (changed || a.ps_out != a.ps || ps != sh_ps[le][t])) a.ps_out[(size_t)t * E + ec] = ps;
     } else if (is_vp) {
+        add_thermal_stress(a, st, t, ec);
         double trace_strain = sn[0] + sn[1] + sn[2];
         double d0, d1, d2;
         devstrain(sn, trace_strain, d0, d1, d2);
         double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
-        double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+        double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt, tet_temp(a, m, nd));
         if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
         maxwell_update(m, st, sn, trace_strain, visc, dt);
         /* unsymmetrised: the lower triangle is zero (viscoplastic_BI :246) */
@@ -756,10 +806,11 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
         }
         double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
         double trace_strain = k.V / old_volume - 1.0;
+        add_thermal_stress(a, st, t, ec);
         double d0, d1, d2;
         devstrain(sn, trace_strain, d0, d1, d2);
         double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
-        double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+        double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt, tet_temp(a, m, nd));
         if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
         maxwell_update(m, st, sn, trace_strain, visc, dt);
     } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
@@ -789,6 +840,7 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
     if (active) {
 #pragma unroll
         for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + ec] = st[c];
+        if (a.dens) store_density(a, m, nd, t, ec);
     }
 }
 
@@ -951,11 +1003,12 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             totalVolume += volt;
         } else if ((RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)) {
             double plasticStrain = a.ps[(size_t)t * E + e];
+            add_thermal_stress(a, st, t, e);
             double trace_strain = sn[0] + sn[1] + sn[2];
             double d0, d1, d2;
             devstrain(sn, trace_strain, d0, d1, d2);
             double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
-            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt, tet_temp_at(a, m, ei, ej, ek, t));
             if (a.visc) a.visc[(size_t)t * E + e] = visc;
             maxwell_update(m, st, sn, trace_strain, visc, dt);
             /* unsymmetrised: the lower triangle is zero (viscoplastic_BI :246) */
@@ -1000,10 +1053,11 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             }
             double old_volume = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
             double trace_strain = volt / old_volume - 1.0;
+            add_thermal_stress(a, st, t, e);
             double d0, d1, d2;
             devstrain(sn, trace_strain, d0, d1, d2);
             double rvisc = a.refvisc ? a.refvisc[e] : m.refvisc;
-            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt);
+            double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt, tet_temp_at(a, m, ei, ej, ek, t));
             if (a.visc) a.visc[(size_t)t * E + e] = visc;
             maxwell_update(m, st, sn, trace_strain, visc, dt);
         } else if ((RSET & RS_ELASTIC) && (m.rheology & SNH_RHEOLOGY_ELASTIC)) {
@@ -1072,6 +1126,8 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             a.diag[3 * E + e] = totalVolume > 0.0 ? depls / totalVolume : 0.0;
         }
     }
+    if (a.dens)
+        for (int t = 0; t < NTET; t++) store_density_at(a, m, ei, ej, ek, t, e);
 }
 
 /* --------------------------------------------------------------------------
@@ -1143,13 +1199,14 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
             const int e = ei + a.EI * (ej + a.EJ * ek);
             const DevMaterial& m = mat_of<MM>(a, e);
             const double inertialDensity = m.lam2mu / (c * c);
-            const double rho = m.rho_eff;
             double X[8][3];
             load_hex(a, a.xc, ei, ej, ek, X);
 #pragma unroll
             for (int idx = 0; idx < 5; idx++) {
                 const int t = kNodeToTetra[slot][idx];
                 const int f = kNodeToSurface[slot][idx];
+                /* effDensity = tetra->density (Force.c:109) */
+                const double rho = a.dens ? a.dens[(size_t)t * E + e] : m.rho_eff;
                 const double* p0 = X[kTetraToNode[t][0]];
                 const double* p1 = X[kTetraToNode[t][1]];
                 const double* p2 = X[kTetraToNode[t][2]];
@@ -1252,6 +1309,174 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_vmax(StepArgs a) {
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+}
+
+/* --------------------------------------------------------------------------
+ * SnacTemperature: the energy entry points
+ * ------------------------------------------------------------------------ */
+
+/* unit normal and area of a triangle, as Tetrahedra_SurfaceNormal /
+   Tetrahedra_SurfaceArea round them (Tetrahedra.c:70-113) */
+__device__ __forceinline__ void face_normal_area(const double* p0, const double* p1, const double* p2, double n[3],
+                                                 double& area) {
+    double ax, ay, az;
+    face_cross(p0, p1, p2, ax, ay, az);
+    const double sq = sqrt(ax * ax + ay * ay + az * az);
+    area = sq / 2.0;
+    const double ysq = 1.0 / sq;
+    n[0] = snh_div_by(ax, sq, ysq);
+    n[1] = snh_div_by(ay, sq, ysq);
+    n[2] = snh_div_by(az, sq, ysq);
+}
+
+/* tet surfaces whose mean normal is the wall normal of the heat-flux
+   condition, per wall i = 0, i = EI-1, k = 0, k = EK-1
+   (temperature/VariableConditions.c:253-308) */
+static constexpr unsigned char kWallFaces[4][4][2] = {
+    {{0, 2}, {1, 2}, {5, 0}, {6, 0}},
+    {{2, 0}, {3, 0}, {7, 0}, {8, 0}},
+    {{0, 1}, {2, 2}, {6, 1}, {7, 1}},
+    {{1, 1}, {3, 2}, {5, 1}, {8, 1}}};
+
+/* 0.25 (n_a + n_b + n_c + n_d) of wall W's four surfaces; the indices are
+   compile-time constants, so X stays in registers */
+template <int W>
+__device__ __forceinline__ void wall_normal(const double X[8][3], double nb[3]) {
+    double nn[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int t = kWallFaces[W][r][0], f = kWallFaces[W][r][1];
+        double area;
+        face_normal_area(X[kTetraToNode[t][kFaceToNode[f][0]]], X[kTetraToNode[t][kFaceToNode[f][1]]],
+                         X[kTetraToNode[t][kFaceToNode[f][2]]], nn[r], area);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) nb[d] = 0.25 * (nn[0][d] + nn[1][d] + nn[2][d] + nn[3][d]);
+}
+
+/* Snac_EP_LoopElementsEnergy: Snac_HeatFlux (HeatFlux.c:63-155) for every
+   tet of the element, then SnacTemperature_HeatFluxBC on the elements of the
+   x and z walls (VariableConditions.c:227-353).  One thread per element. */
+template <bool MM>
+__global__ void __launch_bounds__(GEO_BLOCK) k_heat_elem(StepArgs a) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    const size_t E = (size_t)a.E;
+    const DevMaterial& m = mat_of<MM>(a, e);
+    double X[8][3], T[8];
+    load_hex(a, a.xc, ei, ej, ek, X);
+#pragma unroll
+    for (int q = 0; q < 8; q++) T[q] = a.temp[hex_node_index(a, ei, ej, ek, q)];
+    /* the last matching wall's normal: mean of four tet surface normals */
+    int wall = -1;
+    if (ei == 0) wall = 0;
+    if (ei == a.EI - 1) wall = 1;
+    if (ek == 0) wall = 2;
+    if (ek == a.EK - 1) wall = 3;
+    double nb[3] = {0.0, 0.0, 0.0};
+    switch (wall) {
+    case 0: wall_normal<0>(X, nb); break;
+    case 1: wall_normal<1>(X, nb); break;
+    case 2: wall_normal<2>(X, nb); break;
+    case 3: wall_normal<3>(X, nb); break;
+    default: break;
+    }
+    const double bulkm = m.lambda + snh_div3(2.0 * m.mu);
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        const double* P[4] = {X[kTetraToNode[t][0]], X[kTetraToNode[t][1]], X[kTetraToNode[t][2]],
+                              X[kTetraToNode[t][3]]};
+        const double V = tet_volume(P[0], P[1], P[2], P[3]);
+        const double yV = 1.0 / V;
+        double n[4][3], A[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            face_normal_area(P[kFaceToNode[f][0]], P[kFaceToNode[f][1]], P[kFaceToNode[f][2]], n[f], A[f]);
+        const double T0 = T[kTetraToNode[t][0]], T1 = T[kTetraToNode[t][1]];
+        const double T2 = T[kTetraToNode[t][2]], T3 = T[kTetraToNode[t][3]];
+        double S[3], hf[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            /* node k pairs with surface (k+1)%4, summed left to right */
+            S[d] = T0 * n[1][d] * A[1] + T1 * n[2][d] * A[2] + T2 * n[3][d] * A[3] + T3 * n[0][d] * A[0];
+            hf[d] = snh_div_by(snh_div3(1.0 * m.kcond * S[d]), V, yV);   /* 1.0f*k*(..)/3.0f/V */
+        }
+        if (wall >= 0) {
+            double g[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) g[d] = snh_div_by(snh_div3(-1.0 * S[d]), V, yV);
+            double Fn = 0.0;
+            Fn += g[0] * nb[0];
+            Fn += g[1] * nb[1];
+            Fn += g[2] * nb[2];
+#pragma unroll
+            for (int d = 0; d < 3; d++) hf[d] -= Fn * nb[d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) a.hflux[((size_t)d * NTET + t) * E + e] = hf[d];
+        if (a.tstress) {
+            const int n0 = hex_node_index(a, ei, ej, ek, kTetraToNode[t][0]);
+            const int n1 = hex_node_index(a, ei, ej, ek, kTetraToNode[t][1]);
+            const int n2 = hex_node_index(a, ei, ej, ek, kTetraToNode[t][2]);
+            const int n3 = hex_node_index(a, ei, ej, ek, kTetraToNode[t][3]);
+            const double temp = 0.25 * (T0 + T1 + T2 + T3);
+            const double temp0 = 0.25 * (a.temp0[n0] + a.temp0[n1] + a.temp0[n2] + a.temp0[n3]);
+            a.tstress[(size_t)t * E + e] = -1.0 * m.alpha * bulkm * (temp - temp0);
+        }
+    }
+}
+
+/* Snac_EP_LoopNodesEnergy: Snac_Heat for every node (Heat.c:64-132, source
+   term 0) with the node-centric gather of the force pass, then the
+   temperature before the BCs (read by the tet averages of the next Stress
+   call) and the BCs.  The lumped heat capacity uses the tet densities of the
+   last Stress call; dt_thermal / dt_mech <= 1 raises thermal_err. */
+template <bool MM>
+__global__ void __launch_bounds__(NODE_BLOCK) k_heat_node(StepArgs a) {
+    const int n = xcd_block() * blockDim.x + threadIdx.x;
+    if (n >= a.N) return;
+    const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+    const size_t E = (size_t)a.E;
+    const double dt = a.scal->dt;
+    const double minLen = a.scal->minLen;
+    double energy = 0.0, lumpVolume = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int slot = 0; slot < 8; slot++) {
+        const int ei = i - 1 + (slot & 1), ej = j - 1 + ((slot >> 1) & 1), ek = k - 1 + ((slot >> 2) & 1);
+        if (ei < 0 || ej < 0 || ek < 0 || ei >= a.EI || ej >= a.EJ || ek >= a.EK) continue;
+        const int e = ei + a.EI * (ej + a.EJ * ek);
+        const DevMaterial& m = mat_of<MM>(a, e);
+        double X[8][3];
+        load_hex(a, a.xc, ei, ej, ek, X);
+#pragma unroll
+        for (int idx = 0; idx < 5; idx++) {
+            const int t = kNodeToTetra[slot][idx];
+            const int f = kNodeToSurface[slot][idx];
+            const double V = tet_volume(X[kTetraToNode[t][0]], X[kTetraToNode[t][1]], X[kTetraToNode[t][2]],
+                                        X[kTetraToNode[t][3]]);
+            double nf[3], area;
+            face_normal_area(X[kTetraToNode[t][kFaceToNode[f][0]]], X[kTetraToNode[t][kFaceToNode[f][1]]],
+                             X[kTetraToNode[t][kFaceToNode[f][2]]], nf, area);
+            const double h0 = a.hflux[((size_t)0 * NTET + t) * E + e];
+            const double h1 = a.hflux[((size_t)1 * NTET + t) * E + e];
+            const double h2 = a.hflux[((size_t)2 * NTET + t) * E + e];
+            energy += snh_div6((h0 * nf[0] + h1 * nf[1] + h2 * nf[2]) * area);   /* (..)*area/6.0f */
+            /* the reference's (density == 0 ? formula : density) gives the
+               stored value in both branches when beta = 0 */
+            lumpVolume += m.cp * (a.dens[(size_t)t * E + e] * V) / 8.0;
+        }
+        const double ratio = sqrt(m.lam2mu / m.phsDensity) * minLen / (m.kcond / (m.cp * m.phsDensity));
+        if (!(ratio > 1.0)) bad = true;
+    }
+    const double T = a.temp[n];
+    const double Tn = T + (energy + 0.0) * -1.0 * dt / lumpVolume;
+    a.temp0[n] = T;
+    a.tpre[n] = Tn;
+    const uint8_t code = a.tbc[n];
+    a.temp[n] = code ? a.tbcv[code] : Tn;
+    if (bad) a.scal->thermal_err = 1;
 }
 
 #include "snh_mixed.h"
@@ -1398,6 +1623,17 @@ struct snh_context {
     bool diag_on;             /* steps record dump diagnostics */
     double* diag_buf;         /* [5][E] */
     double* force_buf;        /* [3][N] */
+    /* SnacTemperature (snh_set_thermal) */
+    bool thermal;
+    bool thermal_stress;
+    double* temp;             /* [N] */
+    double* temp0;            /* [N] */
+    double* tpre;             /* [N] */
+    double* hflux;            /* [3][10][E] */
+    double* tstress;          /* [10][E] or null */
+    double* dens;             /* [10][E] */
+    uint8_t* tbc;             /* [N] */
+    double* tbcv;             /* [256] */
     /* prescribed dt / speed of sound / damping branch for steps
        [sched_base, sched_base + sched_n) of timeStep */
     double* sched_dt;
@@ -1413,7 +1649,7 @@ struct snh_context {
     size_t ev_used;
 };
 
-enum { KK_ELEM = 0, KK_NODE = 1, KK_GEO = 2, KK_FIN = 3 };
+enum { KK_ELEM = 0, KK_NODE = 1, KK_GEO = 2, KK_FIN = 3, KK_HEAT_ELEM = 4, KK_HEAT_NODE = 5, KK_KINDS = 6 };
 
 static hipEvent_t take_event(snh_context* c) {
     if (c->ev_used == c->ev_pool.size()) {
@@ -1483,6 +1719,10 @@ static void fill_dev_material(const snh_material& s, DevMaterial& d) {
     d.activationE = s.activationE;
     d.srexponent = s.srexponent;
     d.linear_visc = (s.srexponent == 1.0 && s.activationE == 0.0);
+    d.phsDensity = s.phsDensity;
+    d.alpha = s.alpha;
+    d.kcond = s.thermal_conduct;
+    d.cp = s.heatCapacity;
 }
 
 static bool on_wall(const snh_context* c, int n, int wall) {
@@ -1526,6 +1766,14 @@ static StepArgs make_args(snh_context* c) {
     a.scal = c->scal;
     a.mats = c->dmat;
     a.bcs = c->dbc;
+    a.temp = c->thermal ? c->temp : nullptr;
+    a.temp0 = c->thermal ? c->temp0 : nullptr;
+    a.tpre = c->thermal ? c->tpre : nullptr;
+    a.hflux = c->thermal ? c->hflux : nullptr;
+    a.tstress = c->thermal && c->thermal_stress ? c->tstress : nullptr;
+    a.dens = c->thermal ? c->dens : nullptr;
+    a.tbc = c->thermal ? c->tbc : nullptr;
+    a.tbcv = c->thermal ? c->tbcv : nullptr;
     a.I = c->I; a.J = c->J; a.K = c->K;
     a.EI = c->EI; a.EJ = c->EJ; a.EK = c->EK;
     a.N = c->N; a.E = c->E;
@@ -1539,7 +1787,8 @@ static StepArgs make_args(snh_context* c) {
 static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
                     c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
-                    c->dmat, c->dbc, c->diag_buf, c->force_buf};
+                    c->dmat, c->dbc, c->diag_buf, c->force_buf, c->temp, c->temp0, c->tpre, c->hflux,
+                    c->tstress, c->dens, c->tbc, c->tbcv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1716,6 +1965,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         s0.speedOfSound = 0.0;
         s0.minLen = 0.0;
         s0.vmax = 0.0;
+        s0.thermal_err = 0;
         if (hipMemcpy(c->scal, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_err(SNH_EHIP, "scalar upload failed");
             break;
@@ -1827,6 +2077,10 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
         const double zero = 0.0;
         HIPCHK(hipMemcpy((char*)c->scal + offsetof(DevScalars, dt_rot), &zero, sizeof zero, hipMemcpyHostToDevice));
         c->rotation_pending = false;
+    }
+    if (s->temperature) {
+        if (!c->thermal) return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
+        HIPCHK(hipMemcpy(c->temp, s->temperature, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice));
     }
     if (s->plastic_strain) {
         if (!c->ps) return set_err(SNH_EINVAL, "no plastic rheology in this context");
@@ -1959,6 +2213,111 @@ static int launch_node(snh_context* c) {
     return SNH_OK;
 }
 
+static int launch_heat(snh_context* c, bool elem) {
+    StepArgs a = make_args(c);
+    if (elem) {
+        KTimer kt(c, KK_HEAT_ELEM);
+        if (a.mat)
+            hipLaunchKernelGGL(k_heat_elem<true>, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_heat_elem<false>, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, a);
+    } else {
+        KTimer kt(c, KK_HEAT_NODE);
+        if (a.mat)
+            hipLaunchKernelGGL(k_heat_node<true>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_heat_node<false>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    }
+    HIPCHK(hipGetLastError());
+    return SNH_OK;
+}
+
+/* tet density phsDensity of every tet, as TimeStepZero sets it
+   (Context.c:1009-1017) */
+static int fill_tet_density(snh_context* c) {
+    std::vector<uint8_t> mat(c->E, 0);
+    if (c->mat) HIPCHK(hipMemcpy(mat.data(), c->mat, c->E, hipMemcpyDeviceToHost));
+    std::vector<double> d(10 * (size_t)c->E);
+    for (size_t e = 0; e < (size_t)c->E; e++)
+        for (int t = 0; t < NTET; t++) d[(size_t)t * c->E + e] = c->cfg.materials[mat[e]].phsDensity;
+    HIPCHK(hipMemcpy(c->dens, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+    return SNH_OK;
+}
+
+extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
+    if (!c || !th) return set_err(SNH_EINVAL, "null argument");
+    if (th->nbcs && !th->bcs) return set_err(SNH_EINVAL, "null temperature BC list");
+    if (c->cfg.precision != SNH_PRECISION_FP64)
+        return set_err(SNH_EUNSUPPORTED, "the temperature plugin runs on the FP64 path");
+    if (c->zero_done) return set_err(SNH_ESTATE, "snh_set_thermal after snh_timestep_zero");
+    if (c->adj) return set_err(SNH_ESTATE, "snh_set_thermal during an adjoint recording");
+    for (uint32_t k = 0; k < c->cfg.nmaterials; k++) {
+        const snh_material& m = c->cfg.materials[k];
+        if (!(m.heatCapacity > 0.0) || !(m.phsDensity > 0.0) || !(m.thermal_conduct >= 0.0))
+            return set_err(SNH_EINVAL, "temperature plugin needs heatCapacity > 0, phsDensity > 0, thermal_conduct >= 0");
+    }
+    const size_t N = c->N, E = c->E;
+    if (!c->temp) {
+        HIPCHK(hipMalloc((void**)&c->temp, N * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c->temp0, N * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c->tpre, N * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c->hflux, 30 * E * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c->dens, 10 * E * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&c->tbc, N));
+        HIPCHK(hipMalloc((void**)&c->tbcv, 256 * sizeof(double)));
+        HIPCHK(hipMemset(c->hflux, 0, 30 * E * sizeof(double)));
+    }
+    if (th->computeThermalStress && !c->tstress) {
+        HIPCHK(hipMalloc((void**)&c->tstress, 10 * E * sizeof(double)));
+        HIPCHK(hipMemset(c->tstress, 0, 10 * E * sizeof(double)));
+    }
+    /* _SnacTemperature_InitialConditions (VariableConditions.c:155-212):
+       T0 = T, then the temperatureBCs (later entries win) */
+    std::vector<double> T(N, 0.0), T0;
+    if (th->temperature) T.assign(th->temperature, th->temperature + N);
+    T0 = T;
+    std::vector<uint8_t> code(N, 0);
+    std::vector<double> vals(256, 0.0);
+    int nvals = 1;
+    for (size_t n = 0; n < N; n++) {
+        int b_last = -1;
+        for (uint32_t b = 0; b < th->nbcs; b++)
+            if (on_wall(c, (int)n, th->bcs[b].wall)) b_last = (int)b;
+        if (b_last < 0) continue;
+        const double v = th->bcs[b_last].value;
+        int k = 1;
+        while (k < nvals && memcmp(&vals[k], &v, sizeof v) != 0) k++;
+        if (k == nvals) {
+            if (nvals == 256) return set_err(SNH_EINVAL, "more than 255 distinct temperature BC values");
+            vals[nvals++] = v;
+        }
+        code[n] = (uint8_t)k;
+        T[n] = v;
+    }
+    HIPCHK(hipMemcpy(c->temp, T.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->temp0, T0.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tpre, T.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tbc, code.data(), N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tbcv, vals.data(), 256 * sizeof(double), hipMemcpyHostToDevice));
+    c->thermal = true;
+    c->thermal_stress = th->computeThermalStress != 0;
+    return fill_tet_density(c);
+}
+
+extern "C" int snh_loop_elements_energy(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->thermal) return set_err(SNH_ESTATE, "no temperature plugin (snh_set_thermal)");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_elements_energy before snh_timestep_zero");
+    return launch_heat(c, true);
+}
+
+extern "C" int snh_loop_nodes_energy(snh_context* c) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (!c->thermal) return set_err(SNH_ESTATE, "no temperature plugin (snh_set_thermal)");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_nodes_energy before snh_timestep_zero");
+    return launch_heat(c, false);
+}
+
 extern "C" int snh_hydrostatic_ic(snh_context* c) {
     if (!c) return set_err(SNH_EINVAL, "null context");
     const size_t N = c->N, E = c->E;
@@ -2010,6 +2369,7 @@ extern "C" int snh_timestep_zero(snh_context* c) {
     HIPCHK(hipGetLastError());
     if ((rc = launch_geo(c))) return rc;
     if ((rc = launch_fin(c))) return rc;
+    if (c->thermal && (rc = fill_tet_density(c))) return rc;
     c->zero_done = true;
     c->rotation_pending = true;
     c->timeStep = 0;
@@ -2058,6 +2418,10 @@ extern "C" int snh_step(snh_context* c, int nsteps) {
             hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c->stream, c->scal, c->sched_dt + si,
                                c->sched_c + si);
             HIPCHK(hipGetLastError());
+        }
+        if (c->thermal) {   /* _Snac_Context_Solve, Context.c:1073-1074 */
+            if ((rc = launch_heat(c, true))) return rc;
+            if ((rc = launch_heat(c, false))) return rc;
         }
         if ((rc = launch_elem(c))) return rc;
         if ((rc = launch_node(c))) return rc;
@@ -2138,10 +2502,21 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
     return SNH_OK;
 }
 
+/* the Journal_Firewall of Snac_Heat (temperature/Heat.c:121-126), raised
+   by any step since the last check */
+static int check_thermal(snh_context* c) {
+    if (!c->thermal) return SNH_OK;
+    int32_t err = 0;
+    HIPCHK(hipMemcpy(&err, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof err, hipMemcpyDeviceToHost));
+    if (err) return set_err(SNH_ENUMERIC, "dt_thermal / dt_mech <= 1: check lambda, mu, density, minLengthScale, "
+                                          "thermal_conduct, heatCapacity (temperature/Heat.c:121-126)");
+    return SNH_OK;
+}
+
 extern "C" int snh_synchronize(snh_context* c) {
     if (!c) return set_err(SNH_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
-    return SNH_OK;
+    return check_thermal(c);
 }
 
 extern "C" int snh_get_scalars(snh_context* c, snh_scalars* out) {
@@ -2167,6 +2542,10 @@ extern "C" int snh_download_state(snh_context* c, snh_state* s) {
         HIPCHK(hipMemcpy(buf.data(), c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
         for (size_t n = 0; n < N; n++)
             for (int d = 0; d < 3; d++) s->coord[3 * n + d] = buf[d * N + n];
+    }
+    if (s->temperature) {
+        if (!c->thermal) return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
+        HIPCHK(hipMemcpy(s->temperature, c->temp, N * sizeof(double), hipMemcpyDeviceToHost));
     }
     if (s->velocity) {
         buf.resize(3 * N);
@@ -2258,8 +2637,8 @@ extern "C" int snh_profile(snh_context* c, int enable) {
 extern "C" int snh_get_kernel_times(snh_context* c, snh_kernel_times* out) {
     if (!c || !out) return set_err(SNH_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(c->stream));
-    double ms[4] = {0, 0, 0, 0};
-    uint32_t n[4] = {0, 0, 0, 0};
+    double ms[KK_KINDS] = {0, 0, 0, 0, 0, 0};
+    uint32_t n[KK_KINDS] = {0, 0, 0, 0, 0, 0};
     for (auto& r : c->recs) {
         float t = 0.0f;
         HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
@@ -2268,6 +2647,8 @@ extern "C" int snh_get_kernel_times(snh_context* c, snh_kernel_times* out) {
     }
     out->elem_ms = ms[KK_ELEM]; out->node_ms = ms[KK_NODE]; out->geo_ms = ms[KK_GEO]; out->fin_ms = ms[KK_FIN];
     out->elem_n = n[KK_ELEM]; out->node_n = n[KK_NODE]; out->geo_n = n[KK_GEO]; out->fin_n = n[KK_FIN];
+    out->heat_elem_ms = ms[KK_HEAT_ELEM]; out->heat_node_ms = ms[KK_HEAT_NODE];
+    out->heat_elem_n = n[KK_HEAT_ELEM]; out->heat_node_n = n[KK_HEAT_NODE];
     return SNH_OK;
 }
 
@@ -2436,6 +2817,7 @@ extern "C" int snh_ensemble_create(snh_context* const* members, int32_t n, int32
         if (!c->zero_done) return set_err(SNH_ESTATE, "ensemble member %d before snh_timestep_zero", i);
         if (c->sched_n > 0 || c->profiling || c->adj)
             return set_err(SNH_ESTATE, "ensemble member %d has a schedule, profiling or an adjoint", i);
+        if (c->thermal) return set_err(SNH_EUNSUPPORTED, "ensemble member %d runs the temperature plugin", i);
         for (int j = 0; j < i; j++)
             if (members[j] == c || members[j]->stream == c->stream)
                 return set_err(SNH_EINVAL, "ensemble members need distinct contexts and streams");

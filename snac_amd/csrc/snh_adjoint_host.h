@@ -89,6 +89,7 @@ extern "C" int snh_adjoint_begin(snh_context* c, const snh_adjoint_config* cfg) 
     if (cfg->param != SNH_PARAM_LOG_VISCOSITY && cfg->param != SNH_PARAM_COHESION)
         return set_err(SNH_EINVAL, "unknown adjoint parameter %d", cfg->param);
     if (c->cfg.precision != SNH_PRECISION_FP64) return set_err(SNH_EUNSUPPORTED, "adjoint: FP64 path only");
+    if (c->thermal) return set_err(SNH_EUNSUPPORTED, "adjoint: the energy entry points are not differentiated");
     if (c->cfg.dtType != SNH_DT_DYNAMIC) return set_err(SNH_EUNSUPPORTED, "adjoint: dynamic dt only");
     for (const snh_material& m : c->mats) {
         if (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC)

@@ -62,6 +62,8 @@ struct DevMaterial {
     /* viscosity */
     double vis_min, vis_max, refvisc, refsrate, reftemp, activationE, srexponent;
     int linear_visc;         /* srexponent == 1 and activationE == 0 */
+    /* temperature plugin */
+    double phsDensity, alpha, kcond, cp;
 };
 
 struct DevScalars {
@@ -70,6 +72,7 @@ struct DevScalars {
     double speedOfSound;
     double minLen;
     double vmax;
+    int32_t thermal_err;   /* set when dt_thermal / dt_mech <= 1 (Heat.c:121-126) */
 };
 
 struct DevBC {
@@ -111,6 +114,15 @@ struct StepArgs {
        [3][N] */
     double* diag;
     double* force_out;
+    /* SnacTemperature, all null when the plugin is off */
+    double* temp;      /* [N] nodal temperature                              */
+    double* temp0;     /* [N] nodal temperature of the step before           */
+    double* tpre;      /* [N] temperature of this step before the BCs        */
+    double* hflux;     /* [3][10][E] heat flux                               */
+    double* tstress;   /* [10][E] thermal stress, null unless computed       */
+    double* dens;      /* [10][E] tet density of the last Stress call        */
+    const uint8_t* tbc;   /* [N] temperature-condition code, 0 = none        */
+    const double* tbcv;   /* value of each code                              */
     const DevMaterial* __restrict__ mats;  /* material table of this context */
     const DevBC* __restrict__ bcs;         /* velocity-condition table        */
     double* __restrict__ part_min;

@@ -47,6 +47,8 @@ class MaterialSpec:
     reftemp: float = 1400.0
     activationE: float = 5.4e03
     srexponent: float = 1.0
+    thermal_conduct: float = 2.0
+    heatCapacity: float = 1000.0
 
     def to_c(self) -> _abi.Material:
         m = _abi.Material()
@@ -76,7 +78,41 @@ class MaterialSpec:
         m.reftemp = self.reftemp
         m.activationE = self.activationE
         m.srexponent = self.srexponent
+        m.thermal_conduct = self.thermal_conduct
+        m.heatCapacity = self.heatCapacity
         return m
+
+
+@dataclass
+class ThermalSpec:
+    """The SnacTemperature plugin: initial nodal temperature (nodeICs),
+    temperatureBCs (wall, value; later entries win) and computeThermalStress.
+    top2bottom(problem, top, bottom) gives the reference's
+    _SnacTemperature_Top2BottomSweep initial field (temperature/
+    VariableConditions.c:55-71): T = bottom + (top - bottom) / (J - 1) * j."""
+    temperature: np.ndarray                      # [N]
+    bcs: List[Tuple[str, float]] = field(default_factory=list)
+    computeThermalStress: bool = False
+
+    def to_c(self):
+        T = np.ascontiguousarray(self.temperature, dtype=np.float64)
+        bcs = (_abi.TemperatureBC * max(1, len(self.bcs)))()
+        for idx, (wall, val) in enumerate(self.bcs):
+            bcs[idx].wall = _abi.WALL_NAMES[wall] if isinstance(wall, str) else wall
+            bcs[idx].value = val
+        th = _abi.Thermal()
+        th.computeThermalStress = int(self.computeThermalStress)
+        th.nbcs = len(self.bcs)
+        th.bcs = C.cast(bcs, C.POINTER(_abi.TemperatureBC))
+        th.temperature = T.ctypes.data_as(C.POINTER(C.c_double))
+        return th, (T, bcs)
+
+
+def top2bottom(nodes, top: float = 0.0, bottom: float = 1300.0) -> np.ndarray:
+    I, J, K = nodes
+    n = np.arange(I * J * K)
+    j = (n // I) % J
+    return bottom + (top - bottom) / (J - 1) * j
 
 
 @dataclass
@@ -100,6 +136,7 @@ class Problem:
     fields: Dict[int, np.ndarray] = field(default_factory=dict)   # SNH_FIELD_* -> [E]
     plastic_strain0: Optional[np.ndarray] = None          # [E,10]
     hydrostatic: bool = False      # SnacHydroStaticIC initial stress
+    thermal: Optional["ThermalSpec"] = None   # SnacTemperature plugin
     name: str = "custom"
 
     @property
@@ -313,10 +350,50 @@ def mohr_coulomb_weak_crust(elements=(64, 32, 64), extent_km=(400.0, 100.0, 200.
     return p
 
 
+def maxwell_thermal(elements=(64, 32, 64), extent_km=(100.0, 50.0, 100.0), seed=5, activationE=0.0,
+                    thermal_stress=True) -> Problem:
+    """Maxwell visco-elastic box compression with the SnacTemperature plugin:
+    initial temperature the reference's top-to-bottom sweep (0 to 1300 C)
+    plus a seeded Gaussian anomaly of 200 C (radius 10 km) in the lower
+    half, top and bottom held at 0 and 1300 C, thermal expansion
+    alpha = 3e-5 (density and, with computeThermalStress, thermal stress),
+    conductivity 2 W/m/K, heat capacity 1000 J/kg/K.  activationE = 0 keeps
+    the viscosity linear (exp(0) = 1); a positive value makes it Arrhenius
+    in the tet-average temperature."""
+    lx, ly, lz = (1e3 * e for e in extent_km)
+    v = CM_PER_YEAR
+    extent = ((0.0, lx), (-ly, 0.0), (0.0, lz))
+    mat = _parity_material(rheology=_abi.RHEOLOGY_MAXWELL, activationE=activationE, srexponent=1.0,
+                           refvisc=1.0e21, alpha=f32(3.0e-5), thermal_conduct=2.0, heatCapacity=1000.0)
+    p = _parity_problem(
+        name="maxwell_thermal",
+        elements=tuple(elements),
+        extent=extent,
+        materials=[mat],
+        bcs=_box_bcs(+v),
+        hydrostatic=True,
+    )
+    p.fields[_abi.FIELD_REFVISC] = lognormal_field(elements, extent, seed=seed)
+    I, J, K = p.nodes
+    T = top2bottom(p.nodes, 0.0, 1300.0)
+    rng = splitmix64(seed)
+    cx, cz = (next(rng) % 1000) / 1000.0 * lx, (next(rng) % 1000) / 1000.0 * lz
+    cy = -0.7 * ly
+    n = np.arange(I * J * K)
+    x = (n % I) / (I - 1) * lx
+    y = -ly + ((n // I) % J) / (J - 1) * ly
+    z = (n // (I * J)) / (K - 1) * lz
+    T = T + 200.0 * np.exp(-((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / (1.0e4) ** 2)
+    p.thermal = ThermalSpec(temperature=T, bcs=[("top", 0.0), ("bottom", 1300.0)],
+                            computeThermalStress=thermal_stress)
+    return p
+
+
 WORKLOADS = {
     "elastic_box_compression": elastic_box_compression,
     "mohr_coulomb_extension": mohr_coulomb_extension,
     "maxwell_viscoelastic": maxwell_viscoelastic,
     "mohr_coulomb_shear": mohr_coulomb_shear,
     "mohr_coulomb_weak_crust": mohr_coulomb_weak_crust,
+    "maxwell_thermal": maxwell_thermal,
 }
