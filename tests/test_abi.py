@@ -37,7 +37,7 @@ def test_ctypes_mirror_covers_the_header():
 
 def test_abi_version_and_error_string_without_gpu():
     lib = _abi.load_library()
-    assert lib.snh_abi_version() == 1
+    assert lib.snh_abi_version() == _abi.ABI_VERSION
     h = C.c_void_p()
     rc = lib.snh_context_create(None, C.byref(h))
     assert rc == _abi.SNH_EINVAL
@@ -54,8 +54,10 @@ def test_struct_sizes_match_c():
     with open(src, "w") as f:
         f.write('#include <stdio.h>\n#include "%s"\nint main(){printf("%%zu %%zu %%zu %%zu %%zu %%zu\\n",'
                 'sizeof(snh_material),sizeof(snh_velocity_bc),sizeof(snh_config),sizeof(snh_state),'
-                'sizeof(snh_scalars),sizeof(snh_kernel_times));}\n' % HEADER)
+                'sizeof(snh_scalars),sizeof(snh_kernel_times));'
+                'printf("%%zu %%zu\\n",sizeof(snh_temperature_bc),sizeof(snh_thermal));}\n' % HEADER)
     subprocess.check_call(["gcc", "-o", exe, src])
     sizes = list(map(int, subprocess.check_output([exe]).split()))
     assert sizes == [C.sizeof(_abi.Material), C.sizeof(_abi.VelocityBC), C.sizeof(_abi.Config),
-                     C.sizeof(_abi.State), C.sizeof(_abi.Scalars), C.sizeof(_abi.KernelTimes)]
+                     C.sizeof(_abi.State), C.sizeof(_abi.Scalars), C.sizeof(_abi.KernelTimes),
+                     C.sizeof(_abi.TemperatureBC), C.sizeof(_abi.Thermal)]
