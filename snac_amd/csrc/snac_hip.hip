@@ -1181,7 +1181,7 @@ __global__ void k_volumes(StepArgs a, double* __restrict__ out) {
  * ------------------------------------------------------------------------ */
 #define GETSIGN(val) ((val) < 0 ? -1.0 : (val) > 0 ? +1.0 : 0.0)
 
-template <bool MM>
+template <bool MM, bool TH>
 __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
     const int n = xcd_block() * blockDim.x + threadIdx.x;
@@ -1206,7 +1206,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
                 const int t = kNodeToTetra[slot][idx];
                 const int f = kNodeToSurface[slot][idx];
                 /* effDensity = tetra->density (Force.c:109) */
-                const double rho = a.dens ? a.dens[(size_t)t * E + e] : m.rho_eff;
+                const double rho = TH ? a.dens[(size_t)t * E + e] : m.rho_eff;
                 const double* p0 = X[kTetraToNode[t][0]];
                 const double* p1 = X[kTetraToNode[t][1]];
                 const double* p2 = X[kTetraToNode[t][2]];
@@ -2204,10 +2204,14 @@ static int launch_node(snh_context* c) {
     if (si >= 0 && c->sched_damp) a.damp_sched = c->sched_damp + (size_t)si * c->N;
     if (c->cfg.precision == SNH_PRECISION_MIXED)
         hipLaunchKernelGGL(k_node_mx, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    else if (a.dens && a.mat)
+        hipLaunchKernelGGL((k_node<true, true>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    else if (a.dens)
+        hipLaunchKernelGGL((k_node<false, true>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     else if (a.mat)
-        hipLaunchKernelGGL(k_node<true>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL((k_node<true, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     else
-        hipLaunchKernelGGL(k_node<false>, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL((k_node<false, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     c->cur ^= 1;
     return SNH_OK;

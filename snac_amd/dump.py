@@ -19,6 +19,8 @@ maxwell/ConstructExtensions.c:79, Output.c:62-85):
                          accumulated in float as the reference does
   plStrain.<rank>        float32 per element (plastic laws): volume-averaged plastic strain
   viscosity.<rank>       float32 per element (Maxwell): log10 of the tet-average viscosity
+  temperature.<rank>     float32 per node (temperature plugin, temperature/Output.c:47-75),
+                         the temperature after the step's energy update
 
 Checkpoint files (Context.c:1294-1298, 1393-1434, 1471-1520;
 plastic_BI/Output.c:83-106; maxwell/Output.c:88-119), written when
@@ -29,6 +31,7 @@ isTimeToCheckpoint (Context.c:1545-1561) holds:
   stressTensorCP.<rank>  float32 x 6 per tet (10 per element), for restarting
   plStrainCP.<rank>      float32 per tet (plastic laws)
   viscosityCP.<rank>     float32 per element, as the viscosity dump
+  temperatureCP.<rank>   float32 per node (temperature/Output.c:78-102)
 
 A dump happens after step n when n == 1 or n % dumpEvery == 0
 (isTimeToDump, Context.c:1536-1543); the dump holds the state after the step
@@ -115,6 +118,9 @@ class DumpWriter:
             self.names.append("plStrain")
         if self.maxwell:
             self.names.append("viscosity")
+        self.thermal = p.thermal is not None
+        if self.thermal:
+            self.names.append("temperature")
         self.files = {n: open(self._path(n), "wb") for n in self.names}
         self._cp_time_step_file = open(self._path("checkpointTimeStep"), "w")
         self.cp_names = ["phaseIndexCP", "coordCP", "velCP", "stressTensorCP"]
@@ -122,6 +128,8 @@ class DumpWriter:
             self.cp_names.append("plStrainCP")
         if self.maxwell:
             self.cp_names.append("viscosityCP")
+        if self.thermal:
+            self.cp_names.append("temperatureCP")
         self.cp_files = {n: open(self._path(n), "wb") for n in self.cp_names}
         sc = ctx.scalars()
         with open(os.path.join(outdir, "minLengthScale.0"), "w") as f:
@@ -167,7 +175,8 @@ class DumpWriter:
                 else np.asarray(p.element_material, dtype=np.int32))
 
     def checkpoint(self):
-        st = self.ctx.state(fields=("coord", "velocity", "stress", "plastic_strain", "viscosity"))
+        fields = ("coord", "velocity", "stress", "plastic_strain", "viscosity")
+        st = self.ctx.state(fields=fields + (("temperature",) if self.thermal else ()))
         w = self.cp_files
         w["stressTensorCP"].write(st["stress"].astype(np.float32).tobytes())
         w["coordCP"].write(st["coord"].astype(np.float32).tobytes())
@@ -177,13 +186,16 @@ class DumpWriter:
             w["plStrainCP"].write(st["plastic_strain"].astype(np.float32).tobytes())
         if self.maxwell:
             w["viscosityCP"].write(log10_mean_viscosity(st["viscosity"]).tobytes())
+        if self.thermal:
+            w["temperatureCP"].write(st["temperature"].astype(np.float32).tobytes())
         for f in w.values():
             f.flush()
 
     def dump(self):
         c = self.ctx
         p = c.problem
-        st = c.state(fields=("coord", "velocity", "stress", "plastic_strain", "viscosity", "volume"))
+        fields = ("coord", "velocity", "stress", "plastic_strain", "viscosity", "volume")
+        st = c.state(fields=fields + (("temperature",) if self.thermal else ()))
         dg = c.diagnostics()
         w = self.files
         w["strainRate"].write(dg["strain_rate"].astype(np.float32).tobytes())
@@ -198,6 +210,8 @@ class DumpWriter:
             w["plStrain"].write(dg["plastic_strain"].astype(np.float32).tobytes())
         if self.maxwell:
             w["viscosity"].write(log10_mean_viscosity(st["viscosity"]).tobytes())
+        if self.thermal:
+            w["temperature"].write(st["temperature"].astype(np.float32).tobytes())
         for f in w.values():
             f.flush()
 
@@ -205,8 +219,8 @@ class DumpWriter:
 def read_checkpoint(outdir: str, problem, rank: int = 0) -> list:
     """The checkpoints in outdir, one dict per record: time_step, time, dt
     and the float32 arrays (coord [N,3], velocity [N,3], stress [E,10,6],
-    phase_index [E], plastic_strain [E,10], log10_viscosity [E] when
-    present), e.g. to upload a restart state with Context.set_state."""
+    phase_index [E], plastic_strain [E,10], log10_viscosity [E],
+    temperature [N] when present), e.g. to upload a restart state with Context.set_state."""
     N, E = problem.n_nodes, problem.n_elements
 
     def path(name):
@@ -219,7 +233,8 @@ def read_checkpoint(outdir: str, problem, rank: int = 0) -> list:
             recs.append({"time_step": int(ts), "time": float(t), "dt": float(dt)})
     shapes = {"coordCP": ("coord", np.float32, (N, 3)), "velCP": ("velocity", np.float32, (N, 3)),
               "stressTensorCP": ("stress", np.float32, (E, 10, 6)), "phaseIndexCP": ("phase_index", np.int32, (E,)),
-              "plStrainCP": ("plastic_strain", np.float32, (E, 10)), "viscosityCP": ("log10_viscosity", np.float32, (E,))}
+              "plStrainCP": ("plastic_strain", np.float32, (E, 10)), "viscosityCP": ("log10_viscosity", np.float32, (E,)),
+              "temperatureCP": ("temperature", np.float32, (N,))}
     for name, (key, dt, shape) in shapes.items():
         if not os.path.exists(path(name)):
             continue

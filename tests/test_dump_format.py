@@ -80,3 +80,17 @@ def test_files_from_oracle_state(tmp_path):
     sim = (tmp_path / "sim.0").read_text().split()
     assert sim[:9] == ["4", "3", "5", "1", "1", "1", "4", "3", "5"]
     assert float((tmp_path / "minLengthScale.0").read_text()) > 0
+
+
+def test_temperature_dump_and_checkpoint(tmp_path):
+    p = P.maxwell_thermal(elements=(4, 3, 4))
+    o = OracleContext(p)
+    w = DumpWriter(str(tmp_path), o, dump_every=2, checkpoint_every=3, max_time_steps=4)
+    w.step(4)          # dumps after 1, 2, 4; checkpoints after 3, 4
+    w.close()
+    N = p.n_nodes
+    T = np.fromfile(tmp_path / "temperature.0", dtype=np.float32).reshape(3, N)
+    assert np.array_equal(T[2], o.state()["temperature"].astype(np.float32))
+    assert (tmp_path / "temperatureCP.0").stat().st_size == 2 * 4 * N
+    recs = read_checkpoint(str(tmp_path), p)
+    assert np.array_equal(recs[-1]["temperature"], T[2])

@@ -17,6 +17,7 @@ CASES = {
     "elastic": lambda: P.elastic_box_compression(elements=(6, 3, 6)),
     "mc_shear": lambda: P.mohr_coulomb_shear(elements=(6, 4, 6)),
     "maxwell": lambda: P.maxwell_viscoelastic(elements=(5, 3, 5)),
+    "maxwell_thermal": lambda: P.maxwell_thermal(elements=(5, 3, 5)),
 }
 
 
