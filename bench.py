@@ -189,18 +189,21 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
 
     # dominant kernel and its roofline
-    per = {k: kt[k + "_ms"] / max(1, kt[k + "_n"]) for k in ("elem", "node", "geo", "fin")}
-    dom = max(("elem", "node", "geo"), key=lambda k: per[k])
+    kinds = ("elem", "node", "geo", "fin") + (("heat_elem", "heat_node") if problem.thermal is not None else ())
+    per = {k: kt[k + "_ms"] / max(1, kt[k + "_n"]) for k in kinds}
+    dom = max((k for k in kinds if k != "fin"), key=lambda k: per[k])
     dom_bytes = {"elem": metrics.elem_kernel_bytes_per_element(problem) * E,
                  "node": metrics.node_kernel_bytes_per_element(problem) * E,
-                 "geo": 24.0 * problem.n_nodes}[dom]
+                 "geo": 24.0 * problem.n_nodes,
+                 "heat_elem": metrics.heat_elem_kernel_bytes_per_element(problem) * E,
+                 "heat_node": metrics.heat_node_kernel_bytes_per_element(problem) * E}[dom]
     achieved = dom_bytes / (per[dom] * 1e-3) / 1e9
     lib = _abi.load_library()
     bw = C.c_double(0.0)
     rc = lib.snh_measure_copy_bandwidth(1 << 30, 20, C.byref(bw))
     measured_peak = bw.value if rc == 0 else None
     step_bytes = metrics.step_bytes_per_element(problem)
-    kernel_sum_ms = sum(kt[k + "_ms"] for k in ("elem", "node", "geo", "fin")) / args.steps
+    kernel_sum_ms = sum(kt[k + "_ms"] for k in kinds) / args.steps
 
     out = {
         "metric": "element-steps/s",
@@ -219,7 +222,7 @@ def main():
                    "nodes": list(problem.nodes), "precision": args.precision},
         "roofline": {
             "bound": "hbm",
-            "kernel": {"elem": "k_elem", "node": "k_node", "geo": "k_geo"}[dom],
+            "kernel": "k_" + dom,
             "achieved": achieved,
             "peak": metrics.HBM_PEAK_GBPS,
             "unit": "GB/s",

@@ -196,6 +196,20 @@ int snh_download_state(snh_context* ctx, snh_state* state);
  * computation on the current coordinates, then uploaded. */
 int snh_hydrostatic_ic(snh_context* ctx);
 
+/* Remesher handoff: the remesher stays on the host (remesher_BI,
+ * hooked on AbstractContext_EP_Sync, Snac/plugins/remesher_BI/Register.c:
+ * 127-131; _SnacRemesher_Remesh, Remesh.c:46-186).  Between steps the host
+ * downloads the state (snh_download_state: stress with the last rotation
+ * applied), computes new coordinates and interpolates node and element
+ * values onto them for the same topology, then hands them back here; NULL
+ * fields are kept.  Replaces the copy-back and _SnacRemesher_UpdateElements
+ * (RemeshElements.c:133-151): the coordinates before the remesh become the
+ * previous coordinates (the old_volume of Maxwell's next volumetric
+ * strain), the new mesh's Jaumann rotation with the dt of the last Sync is
+ * applied at the next CalcStresses, and minLengthScale becomes the minimum
+ * of its current value and the new mesh's; dt is not recomputed.  FP64. */
+int snh_remesh(snh_context* ctx, const snh_state* remeshed);
+
 /* One condition of the temperatureBCs composite (a composite of wall
  * conditions, like velocityBCs; later entries win on shared nodes). */
 typedef struct snh_temperature_bc {

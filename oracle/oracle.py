@@ -92,6 +92,7 @@ def lib():
             "orc_record_decisions": (None, [P, C.POINTER(C.c_int8), C.POINTER(C.c_int8)]),
             "orc_get_bc": (I, [P, C.POINTER(C.c_uint8), DP]),
             "orc_set_thermal": (I, [P, C.POINTER(_abi.Thermal)]),
+            "orc_remesh": (I, [P, C.POINTER(_abi.State)]),
             "orc_loop_elements_energy": (I, [P]),
             "orc_loop_nodes_energy": (I, [P]),
             "orc_get_heat": (I, [P, U, U, DP, DP, DP, DP]),
@@ -218,6 +219,20 @@ class Oracle:
                         conv(stress, (self.E, 10, 6)), conv(plastic_strain, (self.E, 10)), None, None,
                         conv(temperature, (self.N,)))
         _check(lib().orc_set_state(self.h, C.byref(st)))
+
+    def remesh(self, coord=None, velocity=None, stress=None, plastic_strain=None, temperature=None):
+        keep = []
+
+        def conv(a, shape):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
+            keep.append(a)
+            return _dp(a)
+        st = _abi.State(conv(coord, (self.N, 3)), conv(velocity, (self.N, 3)),
+                        conv(stress, (self.E, 10, 6)), conv(plastic_strain, (self.E, 10)), None, None,
+                        conv(temperature, (self.N,)))
+        _check(lib().orc_remesh(self.h, C.byref(st)))
 
     # granular functions -------------------------------------------------
     def update_element(self, e):

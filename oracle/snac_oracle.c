@@ -1282,6 +1282,22 @@ int orc_update_node(orc_context* c, uint32_t n) {
     return SNH_OK;
 }
 
+/* _SnacRemesher_Remesh's copy-back and _SnacRemesher_UpdateElements
+   (remesher_BI/Remesh.c:130-158, RemeshElements.c:133-151): the remeshed
+   arrays replace the current ones, then every element is updated on the new
+   mesh (old_volume = volume, new geometry, rotation with the current dt)
+   and minLengthScale keeps the running minimum. */
+int orc_remesh(orc_context* c, const snh_state* s) {
+    int rc = orc_set_state(c, s);
+    if (rc) return rc;
+    for (unsigned e = 0; e < c->E; e++) {
+        double m;
+        orc_update_element(c, e, &m);
+        if (m < c->minLengthScale) c->minLengthScale = m;
+    }
+    return SNH_OK;
+}
+
 /* ---------------------------------------------------------------------------
  * SnacTemperature plugin: the energy entry points
  * ------------------------------------------------------------------------- */

@@ -63,6 +63,7 @@ int orc_get_bc(orc_context* c, uint8_t* mask, double* value);
 int orc_set_element_material(orc_context* c, const uint8_t* mat, size_t nelem);
 int orc_set_element_field(orc_context* c, int field, const double* v, size_t nelem);
 int orc_set_node_temperature(orc_context* c, const double* t, size_t nnode);
+int orc_remesh(orc_context* c, const snh_state* s);                    /* remesher_BI/Remesh.c:130-158 */
 int orc_set_thermal(orc_context* c, const snh_thermal* th);            /* temperature/VariableConditions.c:155-212 */
 int orc_loop_elements_energy(orc_context* c);                          /* temperature/HeatFlux.c:47-155 */
 int orc_loop_nodes_energy(orc_context* c);                             /* temperature/Heat.c:47-172 */

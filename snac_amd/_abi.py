@@ -187,6 +187,7 @@ SIGNATURES = {
     "snh_hydrostatic_ic": (_I, [_P]),
     "snh_timestep_zero": (_I, [_P]),
     "snh_set_thermal": (_I, [_P, C.POINTER(Thermal)]),
+    "snh_remesh": (_I, [_P, C.POINTER(State)]),
     "snh_loop_elements_energy": (_I, [_P]),
     "snh_loop_nodes_energy": (_I, [_P]),
     "snh_calc_stresses": (_I, [_P]),

@@ -244,5 +244,21 @@ class Context:
             setattr(st, name, _dp(a))
         self._check(self.lib.snh_upload_state(self.h, C.byref(st)))
 
+    def remesh(self, coord=None, velocity=None, stress=None, plastic_strain=None, temperature=None):
+        """Hand a host remesher's output back (snh_remesh): new coordinates
+        and the values interpolated onto them; None keeps a field."""
+        keep = []
+        st = _abi.State()
+        for name, a, shape in (("coord", coord, (self.N, 3)), ("velocity", velocity, (self.N, 3)),
+                               ("stress", stress, (self.E, 10, 6)), ("plastic_strain", plastic_strain, (self.E, 10)),
+                               ("temperature", temperature, (self.N,))):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
+            keep.append(a)
+            setattr(st, name, _dp(a))
+        self._check(self.lib.snh_remesh(self.h, C.byref(st)))
+        return self
+
     def _check(self, rc):
         return _abi.check(rc, self.lib)

@@ -1555,6 +1555,17 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvm
     }
 }
 
+/* minLengthScale after a remesh: the running minimum of
+   _SnacRemesher_UpdateElements (remesher_BI/RemeshElements.c:133-151) starts
+   from the current value; dt is not recomputed */
+__global__ void __launch_bounds__(FIN_BLOCK) k_minlen_merge(StepArgs a, int nmin) {
+    __shared__ double sh[FIN_BLOCK / 64];
+    double mn = INFINITY;
+    for (int q = threadIdx.x; q < nmin; q += blockDim.x) mn = fmin(mn, a.part_min[q]);
+    mn = block_min(mn, sh);
+    if (threadIdx.x == 0 && mn < a.scal->minLen) a.scal->minLen = mn;
+}
+
 #include "snh_adjoint.h"
 
 /* prescribed dt and speed of sound of one step (snh_set_schedule) */
@@ -2306,6 +2317,51 @@ extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
     c->thermal = true;
     c->thermal_stress = th->computeThermalStress != 0;
     return fill_tet_density(c);
+}
+
+extern "C" int snh_remesh(snh_context* c, const snh_state* s) {
+    if (!c || !s) return set_err(SNH_EINVAL, "null argument");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_remesh before snh_timestep_zero");
+    if (c->adj) return set_err(SNH_ESTATE, "snh_remesh during an adjoint recording");
+    if (c->cfg.precision != SNH_PRECISION_FP64) return set_err(SNH_EUNSUPPORTED, "snh_remesh: FP64 path only");
+    int rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t N = c->N, E = c->E;
+    /* the Jaumann rotation of the last LoopElementsMomentum, on the old mesh */
+    if (c->rotation_pending) {
+        if (!c->tmp) HIPCHK(hipMalloc((void**)&c->tmp, 60 * E * sizeof(double)));
+        StepArgs a = make_args(c);
+        hipLaunchKernelGGL(k_rotated, dim3((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), dim3(ELEM_BLOCK), 0, c->stream, a,
+                           c->tmp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->stress, c->tmp, 60 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->rotation_pending = false;
+    }
+    /* the remeshed arrays; the coordinates before the remesh become the
+       previous ones, whose tet volumes are the old_volume of the next step */
+    if (s->coord) {
+        HIPCHK(hipMemcpy(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice));
+        std::vector<double> buf(3 * N);
+        for (size_t n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) buf[d * N + n] = s->coord[3 * n + d];
+        HIPCHK(hipMemcpy(c->x[c->cur], buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    snh_state rest = *s;
+    rest.coord = nullptr;
+    rest.viscosity = nullptr;
+    rest.volume = nullptr;
+    if ((rc = snh_upload_state(c, &rest))) return rc;
+    /* _SnacRemesher_UpdateElements: geometry of the new mesh, its rotation
+       with the dt of the last Sync (applied by the next CalcStresses), and the
+       running minimum of the length scale */
+    if ((rc = launch_geo(c))) return rc;
+    StepArgs a = make_args(c);
+    hipLaunchKernelGGL(k_minlen_merge, dim3(1), dim3(FIN_BLOCK), 0, c->stream, a, c->n_geo_blocks);
+    HIPCHK(hipGetLastError());
+    c->rotation_pending = true;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
 }
 
 extern "C" int snh_loop_elements_energy(snh_context* c) {
