@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from snac_amd import Context
+from snac_amd import Context, _abi
 from snac_amd import problem as P
 
 pytestmark = pytest.mark.gpu
@@ -18,8 +18,10 @@ def _host_remesh(p, s, x_init):
     st = s["stress"].reshape(p.elements[2], p.elements[1], p.elements[0], 10, 6)
     sm = st.copy()
     sm[:, :, 1:] = 0.5 * (st[:, :, 1:] + st[:, :, :-1])
-    return {"coord": coord, "velocity": s["velocity"], "stress": sm.reshape(-1, 10, 6),
-            "plastic_strain": s["plastic_strain"]}
+    out = {"coord": coord, "velocity": s["velocity"], "stress": sm.reshape(-1, 10, 6)}
+    if any(m.rheology & (_abi.RHEOLOGY_PLASTIC | _abi.RHEOLOGY_VISCOPLASTIC) for m in p.materials):
+        out["plastic_strain"] = s["plastic_strain"]
+    return out
 
 
 CASES = {
