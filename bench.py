@@ -45,6 +45,11 @@ def make_problem(name, elements=""):
     return P.WORKLOADS[name](**kw)
 
 
+def problem_is_plastic(problem):
+    from snac_amd import _abi
+    return any(m.rheology & (_abi.RHEOLOGY_PLASTIC | _abi.RHEOLOGY_VISCOPLASTIC) for m in problem.materials)
+
+
 def cpu_baseline(problem, seconds, threads, max_steps=200):
     """The CPU oracle (C restatement of the reference step, gcc -O2
     -ffp-contract=off, OpenMP over the element and node loops) timed on this
@@ -182,6 +187,16 @@ def main():
     kt = ctx.kernel_times()
     ctx.profile(False)
     sc = ctx.scalars()
+    # host round trip of the state (what a dump or a host remesh costs): the
+    # step itself never leaves HBM
+    fields = ("coord", "velocity", "stress") + (("plastic_strain",) if problem_is_plastic(problem) else ())
+    t2 = time.perf_counter()
+    st = ctx.state(fields=fields)
+    t3 = time.perf_counter()
+    ctx.set_state(**st)
+    ctx.synchronize()
+    t4 = time.perf_counter()
+    xfer_bytes = sum(a.nbytes for a in st.values())
     ctx.Destroy()
 
     wall = t1 - t0
@@ -240,6 +255,12 @@ def main():
             "kernel_sum_ms_per_step": kernel_sum_ms,
         },
         "final_dt": sc["dt"],
+        "host_transfer": {
+            "fields": list(fields), "bytes": xfer_bytes,
+            "download_ms": 1e3 * (t3 - t2), "upload_ms": 1e3 * (t4 - t3),
+            "value_with_state_round_trip_every_100_steps":
+                E * 100 / (100 * wall / args.steps + (t4 - t2)),
+        },
         "autotune": tune,
     }
 
