@@ -144,8 +144,7 @@ def ensemble_bench(n_models, steps, graph_steps=10):
             "value": total_e * steps / t_ens, "unit": "element-steps/s",
             "sequential_value": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_ens,
             "elem_variants": {"tet_per_lane": variants.count(0), "element_per_lane": variants.count(1),
-                              "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3),
-                              "tet_per_half_wave": variants.count(4)}}
+                              "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3)}}
 
 
 def main():

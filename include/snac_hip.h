@@ -292,13 +292,12 @@ int snh_profile(snh_context* ctx, int enable);
  * elements) on a scratch copy of the current state, a few launches
  * each, and keeps the fastest for this context.  Call after
  * snh_timestep_zero; the state is not modified.  The choice can be forced
- * with the environment variable SNH_ELEM_VARIANT=0|1|2|3|4. */
+ * with the environment variable SNH_ELEM_VARIANT=0|1|2|3. */
 #define SNH_ELEM_TET_PER_LANE            0
 #define SNH_ELEM_ELEMENT_PER_LANE        1
 #define SNH_ELEM_ELEMENT_PER_LANE_LOWREG 2   /* kinematics recomputed, not kept */
 #define SNH_ELEM_TET_PER_WAVE            3   /* 64 elements per block, one wave per tet */
-#define SNH_ELEM_TET_PER_HALF_WAVE       4   /* 32 elements per block, two tets per wave */
-#define SNH_ELEM_VARIANTS                5
+#define SNH_ELEM_VARIANTS                4
 typedef struct snh_tune_report {
     double   elem_ms[SNH_ELEM_VARIANTS];  /* best launch time per variant, ms */
     int32_t  elem_variant;                /* chosen */

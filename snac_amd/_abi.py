@@ -142,8 +142,7 @@ ELEM_TET_PER_LANE = 0
 ELEM_ELEMENT_PER_LANE = 1
 ELEM_ELEMENT_PER_LANE_LOWREG = 2
 ELEM_TET_PER_WAVE = 3
-ELEM_TET_PER_HALF_WAVE = 4
-ELEM_VARIANTS = 5
+ELEM_VARIANTS = 4
 
 
 class TuneReport(C.Structure):

@@ -121,7 +121,6 @@ class Context:
         self._check(self.lib.snh_autotune(self.h, C.byref(r)))
         return {"elem_ms_tet_per_lane": r.elem_ms[0], "elem_ms_element_per_lane": r.elem_ms[1],
                 "elem_ms_element_per_lane_lowreg": r.elem_ms[2], "elem_ms_tet_per_wave": r.elem_ms[3],
-                "elem_ms_tet_per_half_wave": r.elem_ms[4],
                 "elem_variant": r.elem_variant}
 
     def set_elem_variant(self, variant: int):

@@ -619,18 +619,13 @@ struct Soft {
     double phi, psi, coh, hard;
 };
 
-/* TWN = 0: one 64-lane block holds 6 elements x 10 tets.
-   TWN = 64 ("tet per wave"): a 640-thread block holds 64 elements, wave w
+/* TW = false: one 64-lane block holds 6 elements x 10 tets.
+   TW = true ("tet per wave"): a 640-thread block holds 64 elements, wave w
    taking tet w of all of them, so every stress and node load of a wave is
-   over 64 consecutive elements.  TWN = 32: a 320-thread block holds 32
-   elements, each wave two tets of them (two 256 B runs per load); half the
-   block, so two blocks share a CU and one's barriers overlap the other's
-   work. */
-template <int RSET, int TWN, bool MM>
-__global__ void __launch_bounds__(TWN ? 10 * TWN : 64) k_elem(StepArgs a) {
-    constexpr bool TW = TWN != 0;
-    constexpr int TWD = TWN ? TWN : 1;
-    constexpr int NE = TW ? TWN : EPW;       /* elements per block */
+   over 64 consecutive elements. */
+template <int RSET, bool TW, bool MM>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
+    constexpr int NE = TW ? 64 : EPW;       /* elements per block */
     constexpr int NT = TW ? NTET + 1 : NTET; /* padded against LDS bank conflicts */
     __shared__ double sh_a[NE][NT];
     __shared__ double sh_b[NE][NT];
@@ -640,8 +635,8 @@ __global__ void __launch_bounds__(TWN ? 10 * TWN : 64) k_elem(StepArgs a) {
     /* (tet-per-lane) lanes 60..63 carry no tet: they mirror tet 0 of element
        slot 0 but never write LDS or memory */
     const bool in_wave = TW || lane < EPW * NTET;
-    const int le = TW ? (lane % TWD) : (in_wave ? lane / NTET : 0);
-    const int t = TW ? (lane / TWD) : (in_wave ? lane - le * NTET : 0);
+    const int le = TW ? (lane & 63) : (in_wave ? lane / NTET : 0);
+    const int t = TW ? (lane >> 6) : (in_wave ? lane - le * NTET : 0);
     const int e = xcd_block() * NE + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
@@ -2180,23 +2175,19 @@ static int launch_elem_args(snh_context* c, StepArgs a) {
         HIPCHK(hipGetLastError());
         return SNH_OK;
     }
-    const int twn = c->elem_variant == SNH_ELEM_TET_PER_WAVE        ? 64
-                  : c->elem_variant == SNH_ELEM_TET_PER_HALF_WAVE ? 32
-                                                                   : 0;
-    if (twn) {
-        grid = dim3((c->E + twn - 1) / twn);
-        block = dim3(10 * twn);
+    const bool tw = c->elem_variant == SNH_ELEM_TET_PER_WAVE;
+    if (tw) {
+        grid = dim3((c->E + 63) / 64);
+        block = dim3(640);
     }
 #define TL_LAUNCH(R)                                                                          \
     do {                                                                                      \
         if (mm) {                                                                             \
-            if (twn == 64) hipLaunchKernelGGL((k_elem<R, 64, true>), grid, block, 0, c->stream, a); \
-            else if (twn == 32) hipLaunchKernelGGL((k_elem<R, 32, true>), grid, block, 0, c->stream, a); \
-            else hipLaunchKernelGGL((k_elem<R, 0, true>), grid, block, 0, c->stream, a);      \
+            if (tw) hipLaunchKernelGGL((k_elem<R, true, true>), grid, block, 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_elem<R, false, true>), grid, block, 0, c->stream, a);  \
         } else {                                                                              \
-            if (twn == 64) hipLaunchKernelGGL((k_elem<R, 64, false>), grid, block, 0, c->stream, a); \
-            else if (twn == 32) hipLaunchKernelGGL((k_elem<R, 32, false>), grid, block, 0, c->stream, a); \
-            else hipLaunchKernelGGL((k_elem<R, 0, false>), grid, block, 0, c->stream, a);     \
+            if (tw) hipLaunchKernelGGL((k_elem<R, true, false>), grid, block, 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_elem<R, false, false>), grid, block, 0, c->stream, a); \
         }                                                                                     \
     } while (0)
     switch (c->rset) {

@@ -32,7 +32,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("variant", [0, 3, 4])
+@pytest.mark.parametrize("variant", [0, 3])
 def test_remesh_handoff_matches_oracle(case, variant):
     p = CASES[case]()
     c = Context(p).Construct().Build().Initialise()
