@@ -1312,6 +1312,52 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_vmax(StepArgs a) {
 }
 
 /* --------------------------------------------------------------------------
+ * State transfers: the device's rows [C][M] (one row per component, M nodes
+ * or elements) and the host's records [M][C] (the reference's index order),
+ * transposed on the device through a 64-record LDS tile so that both sides
+ * move contiguous runs.  Stress rows are (component q, tet t) = q*10 + t;
+ * its record column is t*6 + q.
+ * ------------------------------------------------------------------------ */
+enum { LAY_PLAIN = 0, LAY_STRESS = 1 };
+
+template <int LAY>
+__device__ __forceinline__ int row_of_col(int col) {
+    return LAY == LAY_STRESS ? (col % 6) * NTET + col / 6 : col;
+}
+
+template <int C, int LAY>
+__global__ void __launch_bounds__(256) k_rows_to_records(const double* __restrict__ in, double* __restrict__ out,
+                                                         size_t M) {
+    __shared__ double tile[C][65];
+    const size_t m0 = (size_t)blockIdx.x * 64;
+    for (int idx = threadIdx.x; idx < C * 64; idx += 256) {
+        const int r = idx >> 6, j = idx & 63;
+        if (m0 + j < M) tile[r][j] = in[(size_t)r * M + m0 + j];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < C * 64; idx += 256) {
+        const int j = idx / C, col = idx - j * C;
+        if (m0 + j < M) out[(m0 + j) * C + col] = tile[row_of_col<LAY>(col)][j];
+    }
+}
+
+template <int C, int LAY>
+__global__ void __launch_bounds__(256) k_records_to_rows(const double* __restrict__ in, double* __restrict__ out,
+                                                         size_t M) {
+    __shared__ double tile[C][65];
+    const size_t m0 = (size_t)blockIdx.x * 64;
+    for (int idx = threadIdx.x; idx < C * 64; idx += 256) {
+        const int j = idx / C, col = idx - j * C;
+        if (m0 + j < M) tile[row_of_col<LAY>(col)][j] = in[(m0 + j) * C + col];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < C * 64; idx += 256) {
+        const int r = idx >> 6, j = idx & 63;
+        if (m0 + j < M) out[(size_t)r * M + m0 + j] = tile[r][j];
+    }
+}
+
+/* --------------------------------------------------------------------------
  * SnacTemperature: the energy entry points
  * ------------------------------------------------------------------------ */
 
@@ -1622,6 +1668,7 @@ struct snh_context {
     DevMaterial* dmat;   /* material table [SNH_MAX_MATERIALS] */
     DevBC* dbc;          /* velocity-condition table [256], entry 0 unused */
     double* tmp;    /* download staging [6][10][E] */
+    double* stage;  /* host-order records of one field, max(60 E, 3 N) doubles */
     int n_geo_blocks, n_node_blocks;
     unsigned rset;  /* rheologies present */
     bool zero_done;
@@ -1799,7 +1846,7 @@ static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
                     c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
                     c->dmat, c->dbc, c->diag_buf, c->force_buf, c->temp, c->temp0, c->tpre, c->hflux,
-                    c->tstress, c->dens, c->tbc, c->tbcv};
+                    c->tstress, c->dens, c->tbc, c->tbcv, c->stage};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -2057,29 +2104,65 @@ extern "C" int snh_set_element_field(snh_context* c, int field, const double* va
     return SNH_OK;
 }
 
+enum { XF_VEC3 = 0, XF_TET = 1, XF_STRESS = 2 };
+
+static int ensure_stage(snh_context* c) {
+    if (!c->stage) {
+        const size_t n = std::max((size_t)60 * c->E, (size_t)3 * c->N);
+        HIPCHK(hipMalloc((void**)&c->stage, n * sizeof(double)));
+    }
+    return SNH_OK;
+}
+
+/* device rows -> host records of one field */
+static int download_rows(snh_context* c, const double* rows, double* host, size_t M, int kind) {
+    int rc;
+    if ((rc = ensure_stage(c))) return rc;
+    const dim3 grid((unsigned)((M + 63) / 64)), block(256);
+    size_t C;
+    switch (kind) {
+    case XF_VEC3: C = 3; hipLaunchKernelGGL((k_rows_to_records<3, LAY_PLAIN>), grid, block, 0, c->stream, rows, c->stage, M); break;
+    case XF_TET: C = 10; hipLaunchKernelGGL((k_rows_to_records<10, LAY_PLAIN>), grid, block, 0, c->stream, rows, c->stage, M); break;
+    default: C = 60; hipLaunchKernelGGL((k_rows_to_records<60, LAY_STRESS>), grid, block, 0, c->stream, rows, c->stage, M); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, c->stage, C * M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+/* host records -> device rows of one field */
+static int upload_rows(snh_context* c, const double* host, double* rows, size_t M, int kind) {
+    int rc;
+    if ((rc = ensure_stage(c))) return rc;
+    const size_t C = kind == XF_VEC3 ? 3 : kind == XF_TET ? 10 : 60;
+    HIPCHK(hipMemcpyAsync(c->stage, host, C * M * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const dim3 grid((unsigned)((M + 63) / 64)), block(256);
+    switch (kind) {
+    case XF_VEC3: hipLaunchKernelGGL((k_records_to_rows<3, LAY_PLAIN>), grid, block, 0, c->stream, c->stage, rows, M); break;
+    case XF_TET: hipLaunchKernelGGL((k_records_to_rows<10, LAY_PLAIN>), grid, block, 0, c->stream, c->stage, rows, M); break;
+    default: hipLaunchKernelGGL((k_records_to_rows<60, LAY_STRESS>), grid, block, 0, c->stream, c->stage, rows, M); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
 extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
     const size_t N = c->N, E = c->E;
-    std::vector<double> buf;
+    int rc;
+    if (s->temperature && !c->thermal)
+        return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
+    if (s->plastic_strain && !c->ps) return set_err(SNH_EINVAL, "no plastic rheology in this context");
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (s->coord) {
-        buf.resize(3 * N);
-        for (size_t n = 0; n < N; n++)
-            for (int d = 0; d < 3; d++) buf[d * N + n] = s->coord[3 * n + d];
-        HIPCHK(hipMemcpy(c->x[c->cur], buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->x[c->cur ^ 1], buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = upload_rows(c, s->coord, c->x[c->cur], N, XF_VEC3))) return rc;
+        HIPCHK(hipMemcpy(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice));
     }
-    if (s->velocity) {
-        buf.resize(3 * N);
-        for (size_t n = 0; n < N; n++)
-            for (int d = 0; d < 3; d++) buf[d * N + n] = s->velocity[3 * n + d];
-        HIPCHK(hipMemcpy(c->v, buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
-    }
+    if (s->velocity && (rc = upload_rows(c, s->velocity, c->v, N, XF_VEC3))) return rc;
     if (s->stress) {
-        buf.resize(60 * E);
-        for (size_t e = 0; e < E; e++)
-            for (int t = 0; t < NTET; t++)
-                for (int q = 0; q < 6; q++) buf[((size_t)q * NTET + t) * E + e] = s->stress[(e * NTET + t) * 6 + q];
-        HIPCHK(hipMemcpy(c->stress, buf.data(), 60 * E * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = upload_rows(c, s->stress, c->stress, E, XF_STRESS))) return rc;
         c->rotation_pending = false;
     }
     if ((s->coord || s->velocity || s->stress) && c->zero_done) {
@@ -2089,17 +2172,9 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
         HIPCHK(hipMemcpy((char*)c->scal + offsetof(DevScalars, dt_rot), &zero, sizeof zero, hipMemcpyHostToDevice));
         c->rotation_pending = false;
     }
-    if (s->temperature) {
-        if (!c->thermal) return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
+    if (s->temperature)
         HIPCHK(hipMemcpy(c->temp, s->temperature, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice));
-    }
-    if (s->plastic_strain) {
-        if (!c->ps) return set_err(SNH_EINVAL, "no plastic rheology in this context");
-        buf.resize(10 * E);
-        for (size_t e = 0; e < E; e++)
-            for (int t = 0; t < NTET; t++) buf[(size_t)t * E + e] = s->plastic_strain[e * NTET + t];
-        HIPCHK(hipMemcpy(c->ps, buf.data(), 10 * E * sizeof(double), hipMemcpyHostToDevice));
-    }
+    if (s->plastic_strain && (rc = upload_rows(c, s->plastic_strain, c->ps, E, XF_TET))) return rc;
     return SNH_OK;
 }
 
@@ -2342,10 +2417,7 @@ extern "C" int snh_remesh(snh_context* c, const snh_state* s) {
        previous ones, whose tet volumes are the old_volume of the next step */
     if (s->coord) {
         HIPCHK(hipMemcpy(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice));
-        std::vector<double> buf(3 * N);
-        for (size_t n = 0; n < N; n++)
-            for (int d = 0; d < 3; d++) buf[d * N + n] = s->coord[3 * n + d];
-        HIPCHK(hipMemcpy(c->x[c->cur], buf.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = upload_rows(c, s->coord, c->x[c->cur], N, XF_VEC3))) return rc;
     }
     snh_state rest = *s;
     rest.coord = nullptr;
@@ -2595,24 +2667,13 @@ extern "C" int snh_get_scalars(snh_context* c, snh_scalars* out) {
 extern "C" int snh_download_state(snh_context* c, snh_state* s) {
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
     const size_t N = c->N, E = c->E;
-    std::vector<double> buf;
+    int rc;
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (s->coord) {
-        buf.resize(3 * N);
-        HIPCHK(hipMemcpy(buf.data(), c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t n = 0; n < N; n++)
-            for (int d = 0; d < 3; d++) s->coord[3 * n + d] = buf[d * N + n];
-    }
-    if (s->temperature) {
-        if (!c->thermal) return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
-        HIPCHK(hipMemcpy(s->temperature, c->temp, N * sizeof(double), hipMemcpyDeviceToHost));
-    }
-    if (s->velocity) {
-        buf.resize(3 * N);
-        HIPCHK(hipMemcpy(buf.data(), c->v, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t n = 0; n < N; n++)
-            for (int d = 0; d < 3; d++) s->velocity[3 * n + d] = buf[d * N + n];
-    }
+    if (s->temperature && !c->thermal)
+        return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
+    if (s->coord && (rc = download_rows(c, c->x[c->cur], s->coord, N, XF_VEC3))) return rc;
+    if (s->temperature) HIPCHK(hipMemcpy(s->temperature, c->temp, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (s->velocity && (rc = download_rows(c, c->v, s->velocity, N, XF_VEC3))) return rc;
     if (s->stress || s->volume) {
         if (!c->tmp) HIPCHK(hipMalloc((void**)&c->tmp, 60 * E * sizeof(double)));
     }
@@ -2623,41 +2684,26 @@ extern "C" int snh_download_state(snh_context* c, snh_state* s) {
             hipLaunchKernelGGL(k_rotated, dim3((c->E + ELEM_BLOCK - 1) / ELEM_BLOCK), dim3(ELEM_BLOCK), 0, c->stream,
                                a, c->tmp);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(c->stream));
             src = c->tmp;
         }
-        buf.resize(60 * E);
-        HIPCHK(hipMemcpy(buf.data(), src, 60 * E * sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t e = 0; e < E; e++)
-            for (int t = 0; t < NTET; t++)
-                for (int q = 0; q < 6; q++) s->stress[(e * NTET + t) * 6 + q] = buf[((size_t)q * NTET + t) * E + e];
+        if ((rc = download_rows(c, src, s->stress, E, XF_STRESS))) return rc;
     }
     if (s->volume) {
         StepArgs a = make_args(c);
         hipLaunchKernelGGL(k_volumes, dim3((c->E + 127) / 128), dim3(128), 0, c->stream, a, c->tmp);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->stream));
-        buf.resize(10 * E);
-        HIPCHK(hipMemcpy(buf.data(), c->tmp, 10 * E * sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t e = 0; e < E; e++)
-            for (int t = 0; t < NTET; t++) s->volume[e * NTET + t] = buf[(size_t)t * E + e];
+        if ((rc = download_rows(c, c->tmp, s->volume, E, XF_TET))) return rc;
     }
     if (s->plastic_strain) {
         if (c->ps) {
-            buf.resize(10 * E);
-            HIPCHK(hipMemcpy(buf.data(), c->ps, 10 * E * sizeof(double), hipMemcpyDeviceToHost));
-            for (size_t e = 0; e < E; e++)
-                for (int t = 0; t < NTET; t++) s->plastic_strain[e * NTET + t] = buf[(size_t)t * E + e];
+            if ((rc = download_rows(c, c->ps, s->plastic_strain, E, XF_TET))) return rc;
         } else {
             memset(s->plastic_strain, 0, 10 * E * sizeof(double));
         }
     }
     if (s->viscosity) {
         if (c->visc) {
-            buf.resize(10 * E);
-            HIPCHK(hipMemcpy(buf.data(), c->visc, 10 * E * sizeof(double), hipMemcpyDeviceToHost));
-            for (size_t e = 0; e < E; e++)
-                for (int t = 0; t < NTET; t++) s->viscosity[e * NTET + t] = buf[(size_t)t * E + e];
+            if ((rc = download_rows(c, c->visc, s->viscosity, E, XF_TET))) return rc;
         } else {
             /* linear law: the viscosity is the clamped reference viscosity */
             std::vector<double> rv;
