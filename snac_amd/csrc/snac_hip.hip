@@ -2158,7 +2158,11 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
     HIPCHK(hipStreamSynchronize(c->stream));
     if (s->coord) {
         if ((rc = upload_rows(c, s->coord, c->x[c->cur], N, XF_VEC3))) return rc;
-        HIPCHK(hipMemcpy(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice));
+        /* on the context's stream: a device-to-device hipMemcpy may still run
+           when the host returns, unordered with that (non-blocking) stream */
+        HIPCHK(hipMemcpyAsync(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     if (s->velocity && (rc = upload_rows(c, s->velocity, c->v, N, XF_VEC3))) return rc;
     if (s->stress) {
@@ -2416,7 +2420,8 @@ extern "C" int snh_remesh(snh_context* c, const snh_state* s) {
     /* the remeshed arrays; the coordinates before the remesh become the
        previous ones, whose tet volumes are the old_volume of the next step */
     if (s->coord) {
-        HIPCHK(hipMemcpy(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpyAsync(c->x[c->cur ^ 1], c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
         if ((rc = upload_rows(c, s->coord, c->x[c->cur], N, XF_VEC3))) return rc;
     }
     snh_state rest = *s;
