@@ -1181,12 +1181,39 @@ __global__ void k_volumes(StepArgs a, double* __restrict__ out) {
  * ------------------------------------------------------------------------ */
 #define GETSIGN(val) ((val) < 0 ? -1.0 : (val) > 0 ? +1.0 : 0.0)
 
+/* node handled by gather thread g.  Rows of I nodes are taken in bands of
+   node_band k-planes, j outer and k inner within a band: the four node rows
+   that read one element row's stress ((j,k), (j+1,k), (j,k+1), (j+1,k+1))
+   then run close together in time on one XCD and share its L2, where the
+   plain order puts a whole k-plane (J rows) between them.  node_band = 0
+   keeps the plain order.  A bijection of [0, N). */
+__device__ __forceinline__ int gather_node(const StepArgs& a, int g) {
+    const int KB = a.node_band;
+    if (KB <= 0) return g;
+    const int r = g / a.I, i = g - r * a.I;
+    const int per = a.J * KB;
+    const int nfull = a.K / KB;
+    int j, k;
+    if (r < nfull * per) {
+        const int b = r / per, rem = r - b * per;
+        j = rem / KB;
+        k = b * KB + (rem - j * KB);
+    } else {
+        const int kl = a.K - nfull * KB;
+        const int rem = r - nfull * per;
+        j = rem / kl;
+        k = nfull * KB + (rem - j * kl);
+    }
+    return i + a.I * (j + a.J * k);
+}
+
 template <bool MM, bool TH>
 __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
-    const int n = xcd_block() * blockDim.x + threadIdx.x;
+    const int g = xcd_block() * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
-    if (n < a.N) {
+    if (g < a.N) {
+        const int n = gather_node(a, g);
         const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
         const double dt = a.scal->dt;
         const double c = a.scal->speedOfSound;
@@ -1480,8 +1507,9 @@ __global__ void __launch_bounds__(GEO_BLOCK) k_heat_elem(StepArgs a) {
    last Stress call; dt_thermal / dt_mech <= 1 raises thermal_err. */
 template <bool MM>
 __global__ void __launch_bounds__(NODE_BLOCK) k_heat_node(StepArgs a) {
-    const int n = xcd_block() * blockDim.x + threadIdx.x;
-    if (n >= a.N) return;
+    const int g = xcd_block() * blockDim.x + threadIdx.x;
+    if (g >= a.N) return;
+    const int n = gather_node(a, g);
     const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
     const size_t E = (size_t)a.E;
     const double dt = a.scal->dt;
@@ -1677,6 +1705,7 @@ struct snh_context {
     bool needs_visc;
     uint32_t timeStep;
     int elem_variant;   /* SNH_ELEM_* */
+    int node_band;      /* StepArgs::node_band */
     struct AdjointCtx* adj;   /* recorded adjoint run, or null */
     bool diag_on;             /* steps record dump diagnostics */
     double* diag_buf;         /* [5][E] */
@@ -1836,6 +1865,7 @@ static StepArgs make_args(snh_context* c) {
     a.EI = c->EI; a.EJ = c->EJ; a.EK = c->EK;
     a.N = c->N; a.E = c->E;
     a.dtType = c->cfg.dtType;
+    a.node_band = c->node_band;
     a.gravity = c->cfg.gravity;
     a.demf = c->cfg.demf;
     a.strain_inert = c->cfg.strain_inert;
@@ -2037,6 +2067,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     }
     c->cur = 0;
     c->elem_variant = SNH_ELEM_TET_PER_LANE;
+    c->node_band = 8;
+    if (const char* nb = getenv("SNH_NODE_BAND")) c->node_band = atoi(nb);
     c->adj = nullptr;
     c->diag_on = false;
     c->diag_buf = c->force_buf = nullptr;

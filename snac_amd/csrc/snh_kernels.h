@@ -132,6 +132,7 @@ struct StepArgs {
     int32_t EI, EJ, EK;     /* element counts */
     int32_t N, E;
     int32_t dtType;
+    int32_t node_band;      /* k-planes per band of the node passes' order, 0 = plain */
     double gravity, demf, strain_inert;
 };
 
