@@ -203,3 +203,23 @@ def test_autograd_function_gives_adjoint_gradient():
     J.backward()
     _, g, _, _, _ = gpu_gradient(p, param, n, interval, target)
     assert np.array_equal(theta.grad.numpy(), g)
+
+
+def test_steps_after_a_recorded_forward_match_plain_steps():
+    """The recorded forward leaves the context's state as plain steps do:
+    stepping on afterwards gives the same run."""
+    make, param, n, interval = CASES["mc_shear"]
+    p = make()
+    a = Context(p).Construct().Build().Initialise()
+    a.adjoint_begin(param, interval)
+    a.adjoint_forward(n)
+    a.adjoint_end()
+    a.Execute(7)
+    sa = a.state()
+    a.Destroy()
+    b = Context(p).Construct().Build().Initialise()
+    b.Execute(n + 7)
+    sb = b.state()
+    b.Destroy()
+    for f in ("coord", "velocity", "stress", "plastic_strain"):
+        assert np.array_equal(sa[f], sb[f]), f
