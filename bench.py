@@ -50,11 +50,13 @@ def problem_is_plastic(problem):
     return any(m.rheology & (_abi.RHEOLOGY_PLASTIC | _abi.RHEOLOGY_VISCOPLASTIC) for m in problem.materials)
 
 
-def cpu_baseline(problem, seconds, threads, max_steps=200):
+def cpu_baseline(problem, seconds, threads, max_steps=200, st_steps=1):
     """The CPU oracle (C restatement of the reference step, gcc -O2
     -ffp-contract=off, OpenMP over the element and node loops) timed on this
-    host: TimeStepZero, one warm-up step, then whole steps until `seconds` of
-    wall time have passed (at least two).  Returns (rate, seconds, steps)."""
+    host: TimeStepZero, one warm-up step, then whole steps on `threads`
+    threads until `seconds` of wall time have passed (at least two), then
+    `st_steps` steps on one thread.  Returns ((rate, seconds, steps) for the
+    threads, (rate, seconds, steps) for one thread)."""
     from oracle.oracle import Oracle
     o = Oracle(problem, threads=threads)
     if problem.hydrostatic:
@@ -69,8 +71,34 @@ def cpu_baseline(problem, seconds, threads, max_steps=200):
         if steps >= 2 and time.perf_counter() - t0 >= seconds:
             break
     t1 = time.perf_counter()
+    Oracle.set_threads(1)
+    t2 = time.perf_counter()
+    o.step(st_steps)
+    t3 = time.perf_counter()
     o.close()
-    return problem.n_elements * steps / (t1 - t0), t1 - t0, steps
+    E = problem.n_elements
+    return (E * steps / (t1 - t0), t1 - t0, steps), (E * st_steps / (t3 - t2), t3 - t2, st_steps)
+
+
+def cpu_bc_scan_cost(steps=200):
+    """The reference applies velocity BCs through a linear search of the BC
+    index table for every node (VariableCondition.c:459-463, O(N B) per
+    step); the oracle's default is an O(1) per-node table.  Both timed on one
+    thread on the 16 x 8 x 16 elastic box."""
+    from oracle.oracle import Oracle
+    from snac_amd import problem as P
+    out = {}
+    for faithful in (False, True):
+        p = P.elastic_box_compression()
+        o = Oracle(p, threads=1)
+        o.set_bc_scan(faithful)
+        o.timestep_zero()
+        t0 = time.perf_counter()
+        o.step(steps)
+        out["linear_scan_ms_per_step" if faithful else "table_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
+        o.close()
+    out["workload"] = "elastic_box_compression (16, 8, 16), one thread"
+    return out
 
 
 def adjoint_bench(make, param, steps, interval, flags=0):
@@ -278,12 +306,15 @@ def main():
 
     if not args.no_cpu and args.cpu_seconds > 0:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        cval, csec, csteps = cpu_baseline(problem, args.cpu_seconds, threads)
+        (cval, csec, csteps), (sval, ssec, ssteps) = cpu_baseline(problem, args.cpu_seconds, threads)
         out["cpu_baseline"] = {
             "value": cval, "unit": "element-steps/s", "cores": threads, "kind": "port",
             "sample": f"{problem.name} {problem.elements} full mesh, {csteps} timed steps after "
                       f"TimeStepZero + 1 warm-up step ({csec:.1f} s), CPU oracle gcc -O2 -ffp-contract=off "
                       f"OpenMP on {cpu_model()}",
+            "nproc": os.cpu_count(),
+            "single_thread": {"value": sval, "steps": ssteps, "seconds": ssec},
+            "bc_application": cpu_bc_scan_cost(),
         }
     print(json.dumps(out), flush=True)
 
