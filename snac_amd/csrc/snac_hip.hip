@@ -650,45 +650,14 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
     int nd[4];
     double Xt[4][3], Ut[4][3];
 #pragma unroll
-    for (int q = 0; q < 4; q++) nd[q] = hex_node_index(a, ei, ej, ek, tet_node(t, q));
-    if constexpr (TW) {
-        /* the 8 corner nodes of the block's 64 elements go through LDS once
-           (coordinates and velocities, [corner][component][element]): each
-           corner is shared by about five of an element's tets, so the tets
-           read them from LDS instead of loading them five times */
-        __shared__ double sh_x[8][6][64];
-        if (lane < 8 * 64) {
-            const int q = lane >> 6, l = lane & 63;
-            const int el = xcd_block() * NE + l;
-            const int ecl = el < a.E ? el : 0;
-            const int n = hex_node_index(a, ecl % a.EI, (ecl / a.EI) % a.EJ, ecl / (a.EI * a.EJ), q);
-            sh_x[q][0][l] = a.xc[n];
-            sh_x[q][1][l] = a.xc[a.N + n];
-            sh_x[q][2][l] = a.xc[2 * a.N + n];
-            sh_x[q][3][l] = a.v[n];
-            sh_x[q][4][l] = a.v[a.N + n];
-            sh_x[q][5][l] = a.v[2 * a.N + n];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int c = tet_node(t, q);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                Xt[q][d] = sh_x[c][d][le];
-                Ut[q][d] = sh_x[c][3 + d][le];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            Xt[q][0] = a.xc[nd[q]];
-            Xt[q][1] = a.xc[a.N + nd[q]];
-            Xt[q][2] = a.xc[2 * a.N + nd[q]];
-            Ut[q][0] = a.v[nd[q]];
-            Ut[q][1] = a.v[a.N + nd[q]];
-            Ut[q][2] = a.v[2 * a.N + nd[q]];
-        }
+    for (int q = 0; q < 4; q++) {
+        nd[q] = hex_node_index(a, ei, ej, ek, tet_node(t, q));
+        Xt[q][0] = a.xc[nd[q]];
+        Xt[q][1] = a.xc[a.N + nd[q]];
+        Xt[q][2] = a.xc[2 * a.N + nd[q]];
+        Ut[q][0] = a.v[nd[q]];
+        Ut[q][1] = a.v[a.N + nd[q]];
+        Ut[q][2] = a.v[2 * a.N + nd[q]];
     }
     TetKin k;
     tet_kinematics(Xt, Ut, k);
