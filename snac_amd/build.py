@@ -15,9 +15,13 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+# roctx ranges around the entry points (visible with rocprofv3 --marker-trace)
+LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"]
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, "snac_hip.hip")]
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + srcs
+    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + srcs + LIBS
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -30,6 +30,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/snac_hip.h"
 #include "snh_kernels.h"
 
@@ -1664,6 +1666,13 @@ static int set_err(int code, const char* fmt, ...) {
     return code;
 }
 
+/* a roctx range over one entry point (rocprofv3 --marker-trace shows them;
+   without a tool attached they cost a function call) */
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+};
+
 #define HIPCHK(call)                                                                              \
     do {                                                                                          \
         hipError_t _e = (call);                                                                   \
@@ -2181,6 +2190,7 @@ static int upload_rows(snh_context* c, const double* host, double* rows, size_t 
 }
 
 extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
+    Range range_("snh_upload_state");
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
     const size_t N = c->N, E = c->E;
     int rc;
@@ -2431,6 +2441,7 @@ extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
 }
 
 extern "C" int snh_remesh(snh_context* c, const snh_state* s) {
+    Range range_("snh_remesh");
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_remesh before snh_timestep_zero");
     if (c->adj) return set_err(SNH_ESTATE, "snh_remesh during an adjoint recording");
@@ -2474,6 +2485,7 @@ extern "C" int snh_remesh(snh_context* c, const snh_state* s) {
 }
 
 extern "C" int snh_loop_elements_energy(snh_context* c) {
+    Range range_("snh_loop_elements_energy");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->thermal) return set_err(SNH_ESTATE, "no temperature plugin (snh_set_thermal)");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_elements_energy before snh_timestep_zero");
@@ -2481,6 +2493,7 @@ extern "C" int snh_loop_elements_energy(snh_context* c) {
 }
 
 extern "C" int snh_loop_nodes_energy(snh_context* c) {
+    Range range_("snh_loop_nodes_energy");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->thermal) return set_err(SNH_ESTATE, "no temperature plugin (snh_set_thermal)");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_nodes_energy before snh_timestep_zero");
@@ -2488,6 +2501,7 @@ extern "C" int snh_loop_nodes_energy(snh_context* c) {
 }
 
 extern "C" int snh_hydrostatic_ic(snh_context* c) {
+    Range range_("snh_hydrostatic_ic");
     if (!c) return set_err(SNH_EINVAL, "null context");
     const size_t N = c->N, E = c->E;
     std::vector<double> x(3 * N), st(60 * E);
@@ -2531,6 +2545,7 @@ extern "C" int snh_hydrostatic_ic(snh_context* c) {
 }
 
 extern "C" int snh_timestep_zero(snh_context* c) {
+    Range range_("snh_timestep_zero");
     if (!c) return set_err(SNH_EINVAL, "null context");
     int rc;
     StepArgs a = make_args(c);
@@ -2547,6 +2562,7 @@ extern "C" int snh_timestep_zero(snh_context* c) {
 }
 
 extern "C" int snh_calc_stresses(snh_context* c) {
+    Range range_("snh_calc_stresses");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_calc_stresses before snh_timestep_zero");
     int rc = launch_elem(c);
@@ -2557,12 +2573,14 @@ extern "C" int snh_calc_stresses(snh_context* c) {
 }
 
 extern "C" int snh_loop_nodes_momentum(snh_context* c) {
+    Range range_("snh_loop_nodes_momentum");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_nodes_momentum before snh_timestep_zero");
     return launch_node(c);
 }
 
 extern "C" int snh_loop_elements_momentum(snh_context* c) {
+    Range range_("snh_loop_elements_momentum");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_elements_momentum before snh_timestep_zero");
     int rc = launch_geo(c);
@@ -2572,11 +2590,13 @@ extern "C" int snh_loop_elements_momentum(snh_context* c) {
 }
 
 extern "C" int snh_sync(snh_context* c) {
+    Range range_("snh_sync");
     if (!c) return set_err(SNH_EINVAL, "null context");
     return launch_fin(c);
 }
 
 extern "C" int snh_step(snh_context* c, int nsteps) {
+    Range range_("snh_step");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_step before snh_timestep_zero");
     if (nsteps < 0) return set_err(SNH_EINVAL, "negative step count");
@@ -2702,6 +2722,7 @@ extern "C" int snh_get_scalars(snh_context* c, snh_scalars* out) {
 }
 
 extern "C" int snh_download_state(snh_context* c, snh_state* s) {
+    Range range_("snh_download_state");
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
     const size_t N = c->N, E = c->E;
     int rc;
@@ -2852,6 +2873,7 @@ extern "C" int snh_set_elem_variant(snh_context* c, int32_t v) {
 }
 
 extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
+    Range range_("snh_autotune");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_autotune before snh_timestep_zero");
     snh_tune_report rep;

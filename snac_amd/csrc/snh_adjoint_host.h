@@ -214,6 +214,7 @@ static int adj_save_checkpoint(snh_context* c, AdjointCtx* A, uint32_t t, const 
 }
 
 extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
+    Range range_("snh_adjoint_forward");
     if (!c) return set_err(SNH_EINVAL, "null context");
     AdjointCtx* A = c->adj;
     if (!A) return set_err(SNH_ESTATE, "snh_adjoint_forward before snh_adjoint_begin");
@@ -292,6 +293,7 @@ static int launch_elem_adj(snh_context* c, const AdjArgs& aa) {
 }
 
 extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, const double* vel_bar, double* grad) {
+    Range range_("snh_adjoint_reverse");
     if (!c || !grad) return set_err(SNH_EINVAL, "null argument");
     AdjointCtx* A = c->adj;
     if (!A) return set_err(SNH_ESTATE, "snh_adjoint_reverse before snh_adjoint_begin");
