@@ -168,11 +168,43 @@ def ensemble_bench(n_models, steps, graph_steps=10):
     t_seq = time.perf_counter() - t0
     for m in members:
         m.Destroy()
-    return {"models": n_models, "elements_total": total_e, "steps": steps, "graph_steps": graph_steps,
-            "value": total_e * steps / t_ens, "unit": "element-steps/s",
-            "sequential_value": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_ens,
-            "elem_variants": {"tet_per_lane": variants.count(0), "element_per_lane": variants.count(1),
-                              "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3)}}
+    out = {"models": n_models, "elements_total": total_e, "steps": steps, "graph_steps": graph_steps,
+           "value": total_e * steps / t_ens, "unit": "element-steps/s",
+           "sequential_value": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_ens,
+           "elem_variants": {"tet_per_lane": variants.count(0), "element_per_lane": variants.count(1),
+                             "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3)}}
+    out["adjoint"] = ensemble_adjoint_bench(probs, steps)
+    return out
+
+
+def ensemble_adjoint_bench(probs, steps, interval=10):
+    """Forward + adjoint of every member (cohesion gradient of a top-surface
+    misfit, corner clamp): members on their own streams from one host thread
+    each, against the members run one after the other."""
+    import numpy as np
+    from snac_amd import Context, _abi
+    from snac_amd.autograd import top_nodes
+    from snac_amd.ensemble import adjoint_gradients
+    rng = np.random.default_rng(7)
+    targets = [1e-3 * rng.standard_normal(len(top_nodes(p))) for p in probs]
+    total_e = sum(p.n_elements for p in probs)
+    members = [Context(p).Construct().Build().Initialise() for p in probs]
+    t0 = time.perf_counter()
+    res = adjoint_gradients(members, _abi.PARAM_COHESION, steps, targets, interval, _abi.ADJ_CORNER_CLAMP)
+    t_con = time.perf_counter() - t0
+    for m in members:
+        m.Destroy()
+    t_seq = 0.0
+    for k, p in enumerate(probs):
+        c = Context(p).Construct().Build().Initialise()
+        t0 = time.perf_counter()
+        adjoint_gradients([c], _abi.PARAM_COHESION, steps, [targets[k]], interval, _abi.ADJ_CORNER_CLAMP, threads=1)
+        t_seq += time.perf_counter() - t0
+        c.Destroy()
+    return {"steps": steps, "checkpoint_interval": interval,
+            "forward_plus_adjoint_element_steps_per_s": total_e * steps / t_con,
+            "sequential_element_steps_per_s": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_con,
+            "finite_gradients": all(bool(np.all(np.isfinite(g))) for _, g in res)}
 
 
 def main():

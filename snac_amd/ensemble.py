@@ -8,11 +8,16 @@ parallel branches.
     ens = Ensemble(members, graph_steps=10, autotune=True)
     ens.step(1000)
     ens.synchronize()
+
+`adjoint_gradients` runs the members' recorded forwards and reverse sweeps
+(snh_adjoint_*) from one host thread per member: every context has its own
+stream, so the members' kernels overlap on the GPU.
 """
 from __future__ import annotations
 
 import ctypes as C
-from typing import List
+from concurrent.futures import ThreadPoolExecutor
+from typing import List, Sequence
 
 import numpy as np
 
@@ -50,6 +55,34 @@ class Ensemble:
             self.destroy()
         except Exception:
             pass
+
+
+def adjoint_gradients(members: Sequence[Context], param: int, nsteps: int, targets: Sequence[np.ndarray],
+                      checkpoint_interval: int = 50, flags: int = 0, threads: int = 0):
+    """For each member (initialised contexts, not yet stepped): the misfit
+    J = 1/2 sum over top nodes of (uplift - target)^2 after nsteps steps and
+    its gradient with respect to the per-element parameter (`param`, an
+    _abi.PARAM_* value), from the member's own adjoint.  Returns a list of
+    (J, gradient) in member order."""
+    from .autograd import top_nodes
+
+    def one(k):
+        c = members[k]
+        top = top_nodes(c.problem)
+        x0 = c.state(fields=("coord",))["coord"]
+        c.adjoint_begin(param, checkpoint_interval, flags)
+        c.adjoint_forward(nsteps)
+        x = c.state(fields=("coord",))["coord"]
+        r = (x[top, 1] - x0[top, 1]) - np.asarray(targets[k], dtype=np.float64)
+        cb = np.zeros((c.N, 3))
+        cb[top, 1] = r
+        g = c.adjoint_reverse(cb)
+        c.adjoint_end()
+        return 0.5 * float(r @ r), g
+
+    n = len(members)
+    with ThreadPoolExecutor(max_workers=threads or min(n, 16)) as ex:
+        return list(ex.map(one, range(n)))
 
 
 def elastoplastic_ensemble(n_models=64, seed=5, nx=(16, 96), ny=(8, 32), phi=(20.0, 35.0),

@@ -79,3 +79,28 @@ def test_uneven_elastoplastic_ensemble_runs():
     ens.destroy()
     for m in members:
         m.Destroy()
+
+
+def test_ensemble_adjoint_equals_member_adjoints():
+    """Members' adjoints run concurrently (one host thread and stream per
+    member) give the same misfits and gradients, bit for bit, as each run
+    alone."""
+    from snac_amd import _abi
+    from snac_amd.autograd import top_nodes
+    from snac_amd.ensemble import adjoint_gradients, elastoplastic_ensemble
+    problems = elastoplastic_ensemble(n_models=4, seed=11, nx=(6, 12), ny=(4, 8))
+    rng = np.random.default_rng(3)
+    targets = [1e-3 * rng.standard_normal(len(top_nodes(p))) for p in problems]
+    members = [Context(p).Construct().Build().Initialise() for p in problems]
+    got = adjoint_gradients(members, _abi.PARAM_COHESION, 20, targets, checkpoint_interval=5,
+                            flags=_abi.ADJ_CORNER_CLAMP)
+    for m in members:
+        m.Destroy()
+    for k, p in enumerate(problems):
+        c = Context(p).Construct().Build().Initialise()
+        (J, g), = adjoint_gradients([c], _abi.PARAM_COHESION, 20, [targets[k]], checkpoint_interval=5,
+                                    flags=_abi.ADJ_CORNER_CLAMP, threads=1)
+        c.Destroy()
+        assert J == got[k][0]
+        assert np.array_equal(g, got[k][1])
+        assert np.any(g != 0.0)
