@@ -87,8 +87,11 @@ def test_ensemble_adjoint_equals_member_adjoints():
     alone."""
     from snac_amd import _abi
     from snac_amd.autograd import top_nodes
-    from snac_amd.ensemble import adjoint_gradients, elastoplastic_ensemble
-    problems = elastoplastic_ensemble(n_models=4, seed=11, nx=(6, 12), ny=(4, 8))
+    from snac_amd.ensemble import adjoint_gradients
+    # uneven sheared Mohr-Coulomb boxes: they yield within a few steps, so
+    # the cohesion gradients are not zero
+    problems = [P.mohr_coulomb_shear(elements=e, seed=k + 1)
+                for k, e in enumerate([(6, 4, 6), (8, 4, 5), (5, 5, 7), (7, 3, 6)])]
     rng = np.random.default_rng(3)
     targets = [1e-3 * rng.standard_normal(len(top_nodes(p))) for p in problems]
     members = [Context(p).Construct().Build().Initialise() for p in problems]
