@@ -1315,6 +1315,10 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
             if (b.mask & 4u) v2 = b.val[2];
             vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
         }
+        /* a NaN or Inf in stress, mass or force reaches the velocities within
+           the step: the counterpart of the reference's release-mode firewall
+           on the element stress (Stress.c:161-166) */
+        if (!(isfinite(v0) && isfinite(v1) && isfinite(v2))) a.scal->nonfinite = 1;
         a.v_out[n] = v0;
         a.v_out[N + n] = v1;
         a.v_out[2 * N + n] = v2;
@@ -1628,6 +1632,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvm
             s->dt = mn * F_045 * a.strain_inert / vmax;
             s->speedOfSound = mn * 0.5 / s->dt;
         }
+        if (!isfinite(s->dt)) s->nonfinite = 1;
     }
 }
 
@@ -2063,6 +2068,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         s0.minLen = 0.0;
         s0.vmax = 0.0;
         s0.thermal_err = 0;
+        s0.nonfinite = 0;
         if (hipMemcpy(c->scal, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_err(SNH_EHIP, "scalar upload failed");
             break;
@@ -2693,19 +2699,21 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
 
 /* the Journal_Firewall of Snac_Heat (temperature/Heat.c:121-126), raised
    by any step since the last check */
-static int check_thermal(snh_context* c) {
-    if (!c->thermal) return SNH_OK;
-    int32_t err = 0;
-    HIPCHK(hipMemcpy(&err, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof err, hipMemcpyDeviceToHost));
-    if (err) return set_err(SNH_ENUMERIC, "dt_thermal / dt_mech <= 1: check lambda, mu, density, minLengthScale, "
-                                          "thermal_conduct, heatCapacity (temperature/Heat.c:121-126)");
+static int check_firewalls(snh_context* c) {
+    int32_t flags[2] = {0, 0};
+    HIPCHK(hipMemcpy(flags, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof flags, hipMemcpyDeviceToHost));
+    if (flags[1]) return set_err(SNH_ENUMERIC, "timeStep %u: dt or minLengthScale is not finite (a NaN or Inf "
+                                               "reached the state; Stress.c:161-166)", c->timeStep);
+    if (c->thermal && flags[0])
+        return set_err(SNH_ENUMERIC, "dt_thermal / dt_mech <= 1: check lambda, mu, density, minLengthScale, "
+                                     "thermal_conduct, heatCapacity (temperature/Heat.c:121-126)");
     return SNH_OK;
 }
 
 extern "C" int snh_synchronize(snh_context* c) {
     if (!c) return set_err(SNH_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
-    return check_thermal(c);
+    return check_firewalls(c);
 }
 
 extern "C" int snh_get_scalars(snh_context* c, snh_scalars* out) {

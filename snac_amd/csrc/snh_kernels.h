@@ -73,6 +73,7 @@ struct DevScalars {
     double minLen;
     double vmax;
     int32_t thermal_err;   /* set when dt_thermal / dt_mech <= 1 (Heat.c:121-126) */
+    int32_t nonfinite;     /* set when a velocity or dt stops being finite */
 };
 
 struct DevBC {

@@ -380,6 +380,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
             if (b.mask & 4u) v2 = b.val[2];
             vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
         }
+        if (!(isfinite(v0) && isfinite(v1) && isfinite(v2))) a.scal->nonfinite = 1;
         a.v_out[n] = v0;
         a.v_out[NN + n] = v1;
         a.v_out[2 * NN + n] = v2;
