@@ -2702,7 +2702,7 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
 static int check_firewalls(snh_context* c) {
     int32_t flags[2] = {0, 0};
     HIPCHK(hipMemcpy(flags, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof flags, hipMemcpyDeviceToHost));
-    if (flags[1]) return set_err(SNH_ENUMERIC, "timeStep %u: dt or minLengthScale is not finite (a NaN or Inf "
+    if (flags[1]) return set_err(SNH_ENUMERIC, "timeStep %u: a velocity or dt is not finite (a NaN or Inf "
                                                "reached the state; Stress.c:161-166)", c->timeStep);
     if (c->thermal && flags[0])
         return set_err(SNH_ENUMERIC, "dt_thermal / dt_mech <= 1: check lambda, mu, density, minLengthScale, "
