@@ -33,31 +33,49 @@
    hex-local index of the node in that element */
 static constexpr unsigned char kSlotHex[8] = {6, 7, 5, 4, 2, 3, 1, 0};
 
-/* for hex node h: the 5 tets holding it, ascending, and the node's local
-   index in each */
-struct HexTets {
-    unsigned char t[8][5];
-    unsigned char q[8][5];
-};
-constexpr HexTets make_hex_tets() {
-    HexTets r{};
-    for (int h = 0; h < 8; h++) {
-        int n = 0;
-        for (int t = 0; t < NTET; t++)
-            for (int q = 0; q < 4; q++)
-                if (kTetraToNode[t][q] == h) {
-                    r.t[h][n] = (unsigned char)t;
-                    r.q[h][n] = (unsigned char)q;
-                    n++;
-                }
+/* LDS slot of tet t's corner q in the per element-node sums: r * 8 + h for
+   the hex node h = kTetraToNode[t][q] and the rank r of t among the 5 tets
+   holding h (ascending).  Stored as [component][r][element][h], the 5 partial
+   values of one element node sit 8 * EPW words apart, and lanes summing
+   consecutive (element node, component) pairs fall in distinct LDS banks
+   for the padded component strides below.  Packed 6 bits per tet, one word
+   per corner. */
+constexpr unsigned long long corner_slot_word(int q) {
+    unsigned long long w = 0;
+    for (int t = 0; t < NTET; t++) {
+        const int h = kTetraToNode[t][q];
+        int r = 0;
+        for (int u = 0; u < t; u++)
+            for (int p = 0; p < 4; p++)
+                if (kTetraToNode[u][p] == h) r++;
+        w |= (unsigned long long)(r * 8 + h) << (6 * t);
     }
-    return r;
+    return w;
 }
-static constexpr HexTets kHexTets = make_hex_tets();
+constexpr bool corner_slots_are_a_bijection() {
+    bool seen[40] = {};
+    for (int q = 0; q < 4; q++)
+        for (int t = 0; t < NTET; t++) {
+            const int s = (int)((corner_slot_word(q) >> (6 * t)) & 63ull);
+            if (s >= 40 || seen[s]) return false;
+            seen[s] = true;
+        }
+    return true;
+}
+static_assert(corner_slots_are_a_bijection(), "each hex node is held by 5 tets");
+__device__ __forceinline__ int corner_slot(int t, int q) {
+    constexpr unsigned long long W[4] = {corner_slot_word(0), corner_slot_word(1), corner_slot_word(2),
+                                         corner_slot_word(3)};
+    const int s = (int)((W[q] >> (6 * t)) & 63ull);
+    return (s >> 3) * (EPW * 8) + (s & 7);
+}
 
 /* number of per element-node adjoint components written by k_elem_adj:
    coordinates (3), velocity (3), coordinates of the previous step (3) */
 #define NBC 9
+/* LDS words per component of the element-node sums in k_elem_adj:
+   5 * EPW * 8 = 240 plus padding against bank conflicts */
+#define ADJ_LDS_STRIDE 249
 
 struct AdjArgs {
     StepArgs f;           /* the forward arguments of the step reversed       */
@@ -90,6 +108,7 @@ struct NodeAdjArgs {
 /* a = (c2 - c1) x (c3 - c1): accumulate d/dc1, d/dc2, d/dc3 from d/da */
 SNH_HD void face_cross_rev(const double* c1, const double* c2, const double* c3, const double ab[3], double* b1,
                            double* b2, double* b3) {
+#pragma clang fp contract(fast)
     const double u0 = c2[0] - c1[0], u1 = c2[1] - c1[1], u2 = c2[2] - c1[2];
     const double w0 = c3[0] - c1[0], w1 = c3[1] - c1[1], w2 = c3[2] - c1[2];
     /* d/du = w x ab, d/dw = ab x u */
@@ -102,6 +121,7 @@ SNH_HD void face_cross_rev(const double* c1, const double* c2, const double* c3,
 
 /* V = |(x3 - x0) . ((x1 - x0) x (x2 - x0))| / 6: accumulate d/dx from d/dV */
 SNH_HD void tet_volume_rev(const double X[4][3], double Vb, double Xb[4][3]) {
+#pragma clang fp contract(fast)
     double a0, a1, a2;
     face_cross(X[0], X[1], X[2], a0, a1, a2);
     const double d0 = X[3][0] - X[0][0], d1 = X[3][1] - X[0][1], d2 = X[3][2] - X[0][2];
@@ -117,6 +137,7 @@ SNH_HD void tet_volume_rev(const double X[4][3], double Vb, double Xb[4][3]) {
    the rotation; b holds d/d(rotated stress) on entry and d/d(stress before)
    on exit; wb accumulates d/dw */
 SNH_HD void rotate_stress_rev(const double s[6], const double w[3], double dt, double b[6], double wb[3]) {
+#pragma clang fp contract(fast)
     const double w01 = w[0], w02 = w[1], w12 = w[2];
     /* forward values after each in-order update */
     const double f0 = s[0] + (2.0 * w01 * s[3] + 2.0 * w02 * s[4]) * dt;
@@ -162,16 +183,13 @@ SNH_HD void rotate_stress_rev(const double s[6], const double w[3], double dt, d
     }
 }
 
-/* reverse of the tet kinematics: from d/d(strain rate) srb (00,11,22,01,02,12),
-   d/d(spin) wb (01,02,12), d/dV Vb and d/d(face cross products) ab,
-   accumulate d/dX and d/dU of the 4 nodes */
-SNH_HD void tet_kinematics_rev(const double X[4][3], const double U[4][3], const TetKin& k, const double srb[6],
-                               const double wb[3], double Vb, double ab[4][3], double Xb[4][3], double Ub[4][3]) {
-    double a[4][3];
-#pragma unroll
-    for (int f = 0; f < 4; f++)
-        face_cross(X[kFaceToNode[f][0]], X[kFaceToNode[f][1]], X[kFaceToNode[f][2]], a[f][0], a[f][1], a[f][2]);
-    /* L from the forward strain rate and spin */
+/* reverse of the tet kinematics, first half: from d/d(strain rate) srb
+   (00,11,22,01,02,12) and d/d(spin) wb (01,02,12), the adjoint of the
+   face-vector sums G (L = -G / (6V), so Gb = -Lb / (6V)) and the direct
+   volume term Vb -= Lb:L / V */
+SNH_HD void tet_kinematics_rev_l(const TetKin& k, const double srb[6], const double wb[3], double& Vb,
+                                 double Gb[3][3]) {
+#pragma clang fp contract(fast)
     double L[3][3], Lb[3][3];
     L[0][0] = k.sr[0]; L[1][1] = k.sr[1]; L[2][2] = k.sr[2];
     L[0][1] = k.sr[3] + k.w[0]; L[1][0] = k.sr[3] - k.w[0];
@@ -181,7 +199,6 @@ SNH_HD void tet_kinematics_rev(const double X[4][3], const double U[4][3], const
     Lb[0][1] = 0.5 * (srb[3] + wb[0]); Lb[1][0] = 0.5 * (srb[3] - wb[0]);
     Lb[0][2] = 0.5 * (srb[4] + wb[1]); Lb[2][0] = 0.5 * (srb[4] - wb[1]);
     Lb[1][2] = 0.5 * (srb[5] + wb[2]); Lb[2][1] = 0.5 * (srb[5] - wb[2]);
-    /* L = -G / (6V) */
     const double V = k.V;
     double LL = 0.0;
 #pragma unroll
@@ -191,6 +208,22 @@ SNH_HD void tet_kinematics_rev(const double X[4][3], const double U[4][3], const
     Vb -= LL / V;
     const double gs = -1.0 / (6.0 * V);
 #pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) Gb[p][q] = gs * Lb[p][q];
+}
+
+/* second half: from Gb, the volume adjoint Vb and d/d(face cross products)
+   ab, accumulate d/dX and d/dU of the 4 nodes (G_pq = sum_k u_kp a_(k+1)%4,q,
+   V = |(x3 - x0) . a0| / 6) */
+SNH_HD void tet_kinematics_rev_x(const double X[4][3], const double U[4][3], const double Gb[3][3], double Vb,
+                                 double ab[4][3], double Xb[4][3], double Ub[4][3]) {
+#pragma clang fp contract(fast)
+    double a[4][3];
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        face_cross(X[kFaceToNode[f][0]], X[kFaceToNode[f][1]], X[kFaceToNode[f][2]], a[f][0], a[f][1], a[f][2]);
+#pragma unroll
     for (int kk = 0; kk < 4; kk++) {
         const int f = (kk + 1) & 3;
 #pragma unroll
@@ -198,14 +231,12 @@ SNH_HD void tet_kinematics_rev(const double X[4][3], const double U[4][3], const
             double ub = 0.0;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                const double Gb = gs * Lb[p][q];
-                ub += Gb * a[f][q];
-                ab[f][q] += Gb * U[kk][p];
+                ub += Gb[p][q] * a[f][q];
+                ab[f][q] += Gb[p][q] * U[kk][p];
             }
             Ub[kk][p] += ub;
         }
     }
-    /* V = |(x3 - x0) . a0| / 6 */
     const double d0 = X[3][0] - X[0][0], d1 = X[3][1] - X[0][1], d2 = X[3][2] - X[0][2];
     const double det = d0 * a[0][0] + d1 * a[0][1] + d2 * a[0][2];
     const double g = (det < 0.0 ? -Vb : Vb) / 6.0;
@@ -256,6 +287,7 @@ SNH_HD MCBranch mc_branch(const DevMaterial& m, const double s[3], double sphi, 
    stress) to Sb and accumulates d/d(cohesion) and d/d(tension cut-off). */
 SNH_HD void mc_return_rev(const DevMaterial& m, const MCBranch& br, const double s[3], const double cn[3][3],
                           const double ob[6], double psb, double Sb[6], double& cohb, double& tcb, bool clamp) {
+#pragma clang fp contract(fast)
     const double a1 = m.lambda + 2.0 * m.mu;
     const double a2 = m.lambda;
     double beta[3], K;
@@ -390,7 +422,12 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_adj(NodeAdjArgs A) {
 /* ------------------------------------------------------------------------
  * k_elem_adj: reverse of the element pass of step n (pending rotation,
  * strain rate, constitutive update, trace averaging) and of the element
- * terms of the node pass (force sigma.a/12, gravity, inertial mass).
+ * terms of the node pass (force sigma.a/12, gravity, inertial mass;
+ * Force.c:112-173).  It recomputes the forward element pass, then reverses
+ * it in two phases that share few registers: the constitutive phase holds
+ * the kinematics and the stresses, the geometry phase loads the node
+ * coordinates and velocities again.  Each phase fits 256 VGPRs, so the
+ * kernel runs two waves per SIMD.
  * ---------------------------------------------------------------------- */
 template <int RSET>
 __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
@@ -398,7 +435,7 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
     __shared__ double sh_a[EPW][NTET];
     __shared__ double sh_b[EPW][NTET];
     __shared__ double sh_sp[EPW][NTET][3];   /* matched, own cohesion, slope */
-    __shared__ double sh_nb[EPW][NTET][4][NBC];
+    __shared__ double sh_nb[NBC * ADJ_LDS_STRIDE];   /* [comp][r][element][h] */
     const int lane = threadIdx.x;
     const bool in_wave = lane < EPW * NTET;
     const int le = in_wave ? lane / NTET : 0;
@@ -429,6 +466,36 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
     }
     TetKin k;
     tet_kinematics(Xt, Ut, k);
+    /* node pass terms of the stress adjoint: f += sigma a_f / 12 at the node
+       opposite face f, f_y -= g rho V / 8, m += (lambda + 2 mu) / c^2 V / 4
+       (Force.c:112-173); taken here, while the coordinates are live */
+    double ob[6], Vb = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) ob[c] = active ? A.sbar[((size_t)c * NTET + t) * E + ec] : 0.0;
+    {
+#pragma clang fp contract(fast)
+        double fa[4][3];
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            face_cross(Xt[kFaceToNode[f][0]], Xt[kFaceToNode[f][1]], Xt[kFaceToNode[f][2]], fa[f][0], fa[f][1],
+                       fa[f][2]);
+        const double gr = a.gravity * m.rho_eff / 8.0;
+        const double md = 0.25 * (m.lam2mu / (csound * csound));
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int f = (q + 1) & 3;
+            const double b0 = A.fbar[nd[q]], b1 = A.fbar[a.N + nd[q]], b2 = A.fbar[2 * a.N + nd[q]];
+            const double mb = A.mbar[nd[q]];
+            const double c12 = 1.0 / 12.0;
+            ob[0] += c12 * b0 * fa[f][0];
+            ob[1] += c12 * b1 * fa[f][1];
+            ob[2] += c12 * b2 * fa[f][2];
+            ob[3] += c12 * (b0 * fa[f][1] + b1 * fa[f][0]);
+            ob[4] += c12 * (b0 * fa[f][2] + b2 * fa[f][0]);
+            ob[5] += c12 * (b1 * fa[f][2] + b2 * fa[f][1]);
+            Vb += -gr * b1 + md * mb;
+        }
+    }
     const double trace = snh_div3(k.sr[0] + k.sr[1] + k.sr[2]);
     if (in_wave) {
         sh_a[le][t] = k.V;
@@ -569,225 +636,229 @@ __global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
 #pragma unroll
     for (int q = 0; q < NTET; q++) pst += sh_b[le][q];
     const double avgTrace = pst / elemVolume;
-    double so[6];   /* sigma*_n of this tet */
-#pragma unroll
-    for (int c = 0; c < 6; c++) so[c] = st[c];
-    so[0] += -1.0 * tr + avgTrace;
-    so[1] += -1.0 * tr + avgTrace;
-    so[2] += -1.0 * tr + avgTrace;
 
-    /* ---------------- reverse ---------------- */
-    double ob[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) ob[c] = active ? A.sbar[((size_t)c * NTET + t) * E + ec] : 0.0;
-    double ab[4][3] = {{0.0}}, Vb = 0.0;
-    /* node pass terms: f += sigma a_f / 12 at the node opposite face f,
-       f_y -= g rho V / 8, m += (lambda + 2 mu) / c^2 V / 4 (Force.c:112-173) */
+    /* ---------------- reverse ----------------
+       (arithmetic of the adjoint only: the forward values above are exact
+       recomputations and keep the uncontracted rounding of the forward) */
     {
-        double fa[4][3];
+#pragma clang fp contract(fast)
+        /* trace averaging: out_ii = s_ii - tr + pst / EV */
+        const double dsum = ob[0] + ob[1] + ob[2];
+        __syncthreads();
+        if (in_wave) sh_b[le][t] = active ? dsum : 0.0;
+        __syncthreads();
+        double avgb = 0.0;
 #pragma unroll
-        for (int f = 0; f < 4; f++)
-            face_cross(Xt[kFaceToNode[f][0]], Xt[kFaceToNode[f][1]], Xt[kFaceToNode[f][2]], fa[f][0], fa[f][1],
-                       fa[f][2]);
-        const double gr = a.gravity * m.rho_eff / 8.0;
-        const double md = 0.25 * (m.lam2mu / (csound * csound));
+        for (int q = 0; q < NTET; q++) avgb += sh_b[le][q];
+        const double pstb = avgb / elemVolume;
+        double EVb = -avgb * avgTrace / elemVolume;
+        double trb = -dsum + pstb * k.V;
+        Vb += pstb * tr;
+        double sg[6];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int f = (q + 1) & 3;
-            const double b0 = A.fbar[nd[q]], b1 = A.fbar[a.N + nd[q]], b2 = A.fbar[2 * a.N + nd[q]];
-            const double mb = A.mbar[nd[q]];
-            const double c12 = 1.0 / 12.0;
-            ob[0] += c12 * b0 * fa[f][0];
-            ob[1] += c12 * b1 * fa[f][1];
-            ob[2] += c12 * b2 * fa[f][2];
-            ob[3] += c12 * (b0 * fa[f][1] + b1 * fa[f][0]);
-            ob[4] += c12 * (b0 * fa[f][2] + b2 * fa[f][0]);
-            ob[5] += c12 * (b1 * fa[f][2] + b2 * fa[f][1]);
-            ab[f][0] += c12 * (so[0] * b0 + so[3] * b1 + so[4] * b2);
-            ab[f][1] += c12 * (so[3] * b0 + so[1] * b1 + so[5] * b2);
-            ab[f][2] += c12 * (so[4] * b0 + so[5] * b1 + so[2] * b2);
-            Vb += -gr * b1 + md * mb;
-        }
-    }
-    /* trace averaging: out_ii = s_ii - tr + pst / EV */
-    const double dsum = ob[0] + ob[1] + ob[2];
-    __syncthreads();
-    if (in_wave) sh_b[le][t] = active ? dsum : 0.0;
-    __syncthreads();
-    double avgb = 0.0;
-#pragma unroll
-    for (int q = 0; q < NTET; q++) avgb += sh_b[le][q];
-    const double pstb = avgb / elemVolume;
-    double EVb = -avgb * avgTrace / elemVolume;
-    double trb = -dsum + pstb * k.V;
-    Vb += pstb * tr;
-    double sg[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) sg[c] = ob[c];
-    sg[0] += trb / 3.0;
-    sg[1] += trb / 3.0;
-    sg[2] += trb / 3.0;
+        for (int c = 0; c < 6; c++) sg[c] = ob[c];
+        sg[0] += trb / 3.0;
+        sg[1] += trb / 3.0;
+        sg[2] += trb / 3.0;
 
-    /* constitutive: d/d(rotated stress) strb, d/d(strain increment) snb */
-    double strb[6], snb[6], parb = 0.0, psb_self = 0.0, cohb = 0.0;
-    if (is_pl) {
-        double psb = active ? A.psbar[(size_t)t * E + ec] : 0.0;
-        psb_self = psb;
-        double Sb[6];
-        if (br.branch) {
-            double tcb = 0.0;
-            mc_return_rev(m, br, d3, cn, sg, psb, Sb, cohb, tcb, (A.flags & SNH_ADJ_CORNER_CLAMP) != 0);
-            if (m.ten_off == 0.0 && phi > 0.0) cohb += tcb / tan_phi(m, phi);
+        /* constitutive: d/d(rotated stress) strb, d/d(strain increment) snb */
+        double strb[6], snb[6], parb = 0.0, psb_self = 0.0, cohb = 0.0;
+        if (is_pl) {
+            double psb = active ? A.psbar[(size_t)t * E + ec] : 0.0;
+            psb_self = psb;
+            double Sb[6];
+            if (br.branch) {
+                double tcb = 0.0;
+                mc_return_rev(m, br, d3, cn, sg, psb, Sb, cohb, tcb, (A.flags & SNH_ADJ_CORNER_CLAMP) != 0);
+                if (m.ten_off == 0.0 && phi > 0.0) cohb += tcb / tan_phi(m, phi);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 6; c++) Sb[c] = sg[c];
+            }
+            const double trS = Sb[0] + Sb[1] + Sb[2];
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                strb[c] = Sb[c];
+                snb[c] = 2.0 * m.mu * Sb[c];
+            }
+            snb[0] += m.lambda * trS;
+            snb[1] += m.lambda * trS;
+            snb[2] += m.lambda * trS;
+            if (A.param == SNH_PARAM_COHESION) parb = cohb;
+        } else if (is_mx) {
+            const double rmu = m.mu;
+            const double bulkm = m.lambda + 2.0 * m.mu / 3.0;
+            const double temp = rmu / (2.0 * visc) * dt;
+            const double vic1 = 1.0 - temp, vic2 = 1.0 / (1.0 + temp);
+            const double ts3 = (st_rot[0] + st_rot[1] + st_rot[2]) / 3.0;
+            const double te3 = te / 3.0;
+            const double VSb = sg[0] + sg[1] + sg[2];
+            double sdb_sum = 0.0, edb_sum = 0.0, thb = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double sd = st_rot[i] - ts3;
+                const double outd = (sd * vic1 + 2.0 * rmu * (sn[i] - te3)) * vic2;
+                const double sdb = sg[i] * vic1 * vic2;
+                const double edb = sg[i] * 2.0 * rmu * vic2;
+                strb[i] = sdb;
+                snb[i] = edb;
+                sdb_sum += sdb;
+                edb_sum += edb;
+                thb += sg[i] * vic2 * (sd + outd) * temp;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) strb[i] += -sdb_sum / 3.0 + VSb / 3.0;
+#pragma unroll
+            for (int c = 3; c < 6; c++) {
+                const double outc = (st_rot[c] * vic1 + 2.0 * rmu * sn[c]) * vic2;
+                strb[c] = sg[c] * vic1 * vic2;
+                snb[c] = sg[c] * 2.0 * rmu * vic2;
+                thb += sg[c] * vic2 * (st_rot[c] + outc) * temp;
+            }
+            const double teb = -edb_sum / 3.0 + bulkm * VSb;
+            /* te = V / V_old - 1 */
+            Vb += teb / old_volume;
+            const double Voldb = -teb * k.V / (old_volume * old_volume);
+            double Xpb[4][3] = {{0.0}};
+            tet_volume_rev(Xp, Voldb, Xpb);
+            if (in_wave) {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) sh_nb[(6 + d) * ADJ_LDS_STRIDE + corner_slot(t, q) + le * 8] = Xpb[q][d];
+            }
+            if (A.param == SNH_PARAM_LOG_VISCOSITY && !clamped) parb = thb;
         } else {
+            const double trS = sg[0] + sg[1] + sg[2];
 #pragma unroll
-            for (int c = 0; c < 6; c++) Sb[c] = sg[c];
+            for (int c = 0; c < 6; c++) {
+                strb[c] = sg[c];
+                snb[c] = 2.0 * m.mu * sg[c];
+            }
+            snb[0] += m.lambda * trS;
+            snb[1] += m.lambda * trS;
+            snb[2] += m.lambda * trS;
         }
-        const double trS = Sb[0] + Sb[1] + Sb[2];
-#pragma unroll
-        for (int c = 0; c < 6; c++) {
-            strb[c] = Sb[c];
-            snb[c] = 2.0 * m.mu * Sb[c];
-        }
-        snb[0] += m.lambda * trS;
-        snb[1] += m.lambda * trS;
-        snb[2] += m.lambda * trS;
-        if (A.param == SNH_PARAM_COHESION) parb = cohb;
-    } else if (is_mx) {
-        const double rmu = m.mu;
-        const double bulkm = m.lambda + 2.0 * m.mu / 3.0;
-        const double temp = rmu / (2.0 * visc) * dt;
-        const double vic1 = 1.0 - temp, vic2 = 1.0 / (1.0 + temp);
-        const double ts3 = (st_rot[0] + st_rot[1] + st_rot[2]) / 3.0;
-        const double te3 = te / 3.0;
-        const double VSb = sg[0] + sg[1] + sg[2];
-        double sdb_sum = 0.0, edb_sum = 0.0, thb = 0.0;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const double sd = st_rot[i] - ts3;
-            const double outd = (sd * vic1 + 2.0 * rmu * (sn[i] - te3)) * vic2;
-            const double sdb = sg[i] * vic1 * vic2;
-            const double edb = sg[i] * 2.0 * rmu * vic2;
-            strb[i] = sdb;
-            snb[i] = edb;
-            sdb_sum += sdb;
-            edb_sum += edb;
-            thb += sg[i] * vic2 * (sd + outd) * temp;
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) strb[i] += -sdb_sum / 3.0 + VSb / 3.0;
-#pragma unroll
-        for (int c = 3; c < 6; c++) {
-            const double outc = (st_rot[c] * vic1 + 2.0 * rmu * sn[c]) * vic2;
-            strb[c] = sg[c] * vic1 * vic2;
-            snb[c] = sg[c] * 2.0 * rmu * vic2;
-            thb += sg[c] * vic2 * (st_rot[c] + outc) * temp;
-        }
-        const double teb = -edb_sum / 3.0 + bulkm * VSb;
-        /* te = V / V_old - 1 */
-        Vb += teb / old_volume;
-        const double Voldb = -teb * k.V / (old_volume * old_volume);
-        double Xpb[4][3] = {{0.0}};
-        tet_volume_rev(Xp, Voldb, Xpb);
-        if (in_wave) {
+        if ((RSET & RS_MAXWELL) && !is_mx && in_wave) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
-                for (int d = 0; d < 3; d++) sh_nb[le][t][q][6 + d] = Xpb[q][d];
+                for (int d = 0; d < 3; d++) sh_nb[(6 + d) * ADJ_LDS_STRIDE + corner_slot(t, q) + le * 8] = 0.0;
         }
-        if (A.param == SNH_PARAM_LOG_VISCOSITY && !clamped) parb = thb;
-    } else {
-        const double trS = sg[0] + sg[1] + sg[2];
+
+        /* strain: sn = (sr + (-trace + srcorr) on the diagonal) dt */
+        double srb[6];
 #pragma unroll
-        for (int c = 0; c < 6; c++) {
-            strb[c] = sg[c];
-            snb[c] = 2.0 * m.mu * sg[c];
+        for (int c = 0; c < 6; c++) srb[c] = snb[c] * dt;
+        const double d6 = srb[0] + srb[1] + srb[2];
+        __syncthreads();
+        if (in_wave) {
+            sh_b[le][t] = active ? d6 : 0.0;
+            sh_a[le][t] = active ? parb : 0.0;
+            sh_sp[le][t][1] = active ? cohb : 0.0;
+            sh_sp[le][t][2] = (double)src;
         }
-        snb[0] += m.lambda * trS;
-        snb[1] += m.lambda * trS;
-        snb[2] += m.lambda * trS;
-    }
-    if (!is_mx && in_wave) {
+        __syncthreads();
+        double srcorrb = 0.0, gsum = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) sh_nb[le][t][q][6 + d] = 0.0;
-    }
+        for (int q = 0; q < NTET; q++) {
+            srcorrb += sh_b[le][q];
+            gsum += sh_a[le][q];
+        }
+        const double partialb = srcorrb / elemVolume;
+        EVb += -srcorrb * srcorr / elemVolume;
+        const double traceb = -d6 + partialb * k.V;
+        Vb += partialb * trace;
+        srb[0] += traceb / 3.0;
+        srb[1] += traceb / 3.0;
+        srb[2] += traceb / 3.0;
+        Vb += EVb;   /* elemVolume = sum of the tet volumes */
 
-    /* strain: sn = (sr + (-trace + srcorr) on the diagonal) dt */
-    double srb[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) srb[c] = snb[c] * dt;
-    const double d6 = srb[0] + srb[1] + srb[2];
-    __syncthreads();
-    if (in_wave) {
-        sh_b[le][t] = active ? d6 : 0.0;
-        sh_a[le][t] = active ? parb : 0.0;
-        sh_sp[le][t][1] = active ? cohb : 0.0;
-        sh_sp[le][t][2] = (double)src;
-    }
-    __syncthreads();
-    double srcorrb = 0.0, gsum = 0.0;
-#pragma unroll
-    for (int q = 0; q < NTET; q++) {
-        srcorrb += sh_b[le][q];
-        gsum += sh_a[le][q];
-    }
-    const double partialb = srcorrb / elemVolume;
-    EVb += -srcorrb * srcorr / elemVolume;
-    const double traceb = -d6 + partialb * k.V;
-    Vb += partialb * trace;
-    srb[0] += traceb / 3.0;
-    srb[1] += traceb / 3.0;
-    srb[2] += traceb / 3.0;
-    Vb += EVb;   /* elemVolume = sum of the tet volumes */
+        /* plastic strain before the step: the identity (ps += dps) plus the
+           softening slope for every tet whose cohesion came from this tet */
+        if (is_pl && active) {
+            double cs = 0.0;
+            for (int q = t; q < NTET; q++)
+                if (sh_sp[le][q][2] == (double)t) cs += sh_sp[le][q][1];
+            A.psbar[(size_t)t * E + ec] = psb_self + own_slope * cs;
+        }
+        if (active && t == 0) A.grad[ec] += gsum;
 
-    /* plastic strain before the step: the identity (ps += dps) plus the
-       softening slope for every tet whose cohesion came from this tet */
-    if (is_pl && active) {
-        double cs = 0.0;
-        for (int q = t; q < NTET; q++)
-            if (sh_sp[le][q][2] == (double)t) cs += sh_sp[le][q][1];
-        A.psbar[(size_t)t * E + ec] = psb_self + own_slope * cs;
-    }
-    if (active && t == 0) A.grad[ec] += gsum;
+        /* rotation of the stored stress */
+        double wbv[3] = {0.0, 0.0, 0.0};
+        double stb[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) stb[c] = strb[c];
+        rotate_stress_rev(st_in, k.w, dt_rot, stb, wbv);
+        if (active) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) A.sbar[((size_t)c * NTET + t) * E + ec] = stb[c];
+        }
 
-    /* rotation of the stored stress */
-    double wbv[3] = {0.0, 0.0, 0.0};
-    double stb[6];
+        /* geometry: the coordinates and velocities are loaded again here
+           rather than held across the constitutive reverse (the empty asm
+           keeps the compiler from reusing the first loads), and sigma*_n is
+           read from the replay tape */
+        double Gb[3][3];
+        tet_kinematics_rev_l(k, srb, wbv, Vb, Gb);
+        int nr[4];
 #pragma unroll
-    for (int c = 0; c < 6; c++) stb[c] = strb[c];
-    rotate_stress_rev(st_in, k.w, dt_rot, stb, wbv);
-    if (active) {
+        for (int q = 0; q < 4; q++) {
+            nr[q] = nd[q];
+            asm volatile("" : "+v"(nr[q]));
+        }
+        double Xr[4][3], Ur[4][3], ab[4][3] = {{0.0}};
 #pragma unroll
-        for (int c = 0; c < 6; c++) A.sbar[((size_t)c * NTET + t) * E + ec] = stb[c];
-    }
-
-    /* geometry */
-    double Xb[4][3] = {{0.0}}, Ub[4][3] = {{0.0}};
-    tet_kinematics_rev(Xt, Ut, k, srb, wbv, Vb, ab, Xb, Ub);
-    if (in_wave) {
+        for (int q = 0; q < 4; q++) {
+            Xr[q][0] = a.xc[nr[q]];
+            Xr[q][1] = a.xc[a.N + nr[q]];
+            Xr[q][2] = a.xc[2 * a.N + nr[q]];
+            Ur[q][0] = a.v[nr[q]];
+            Ur[q][1] = a.v[a.N + nr[q]];
+            Ur[q][2] = a.v[2 * a.N + nr[q]];
+        }
+        {
+            double so[6];   /* sigma*_n of this tet */
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+            for (int c = 0; c < 6; c++) so[c] = active ? a.stress_out[((size_t)c * NTET + t) * E + ec] : 0.0;
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                sh_nb[le][t][q][d] = Xb[q][d];
-                sh_nb[le][t][q][3 + d] = Ub[q][d];
+            for (int q = 0; q < 4; q++) {
+                const int f = (q + 1) & 3;
+                const double b0 = A.fbar[nr[q]], b1 = A.fbar[a.N + nr[q]], b2 = A.fbar[2 * a.N + nr[q]];
+                const double c12 = 1.0 / 12.0;
+                ab[f][0] += c12 * (so[0] * b0 + so[3] * b1 + so[4] * b2);
+                ab[f][1] += c12 * (so[3] * b0 + so[1] * b1 + so[5] * b2);
+                ab[f][2] += c12 * (so[4] * b0 + so[5] * b1 + so[2] * b2);
             }
-    }
-    __syncthreads();
-    /* per element-node sums over the 5 tets at each hex node, ascending tet
-       order; [E][8][NBC] so consecutive lanes store consecutive words */
-    for (int o = lane; o < EPW * 8 * NBC; o += 64) {
-        const int l2 = o / (8 * NBC);
-        const int r = o - l2 * (8 * NBC);
-        const int h = r / NBC;
-        const int comp = r - h * NBC;
-        const int e2 = lb * EPW + l2;
-        if (e2 >= a.E) continue;
-        double sum = 0.0;
+        }
+        double Xb[4][3] = {{0.0}}, Ub[4][3] = {{0.0}};
+        tet_kinematics_rev_x(Xr, Ur, Gb, Vb, ab, Xb, Ub);
+        if (in_wave) {
 #pragma unroll
-        for (int idx = 0; idx < 5; idx++) sum += sh_nb[l2][kHexTets.t[h][idx]][kHexTets.q[h][idx]][comp];
-        A.nb[((size_t)e2 * 8 + h) * NBC + comp] = sum;
+            for (int q = 0; q < 4; q++) {
+                const int sl = corner_slot(t, q) + le * 8;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    sh_nb[d * ADJ_LDS_STRIDE + sl] = Xb[q][d];
+                    sh_nb[(3 + d) * ADJ_LDS_STRIDE + sl] = Ub[q][d];
+                }
+            }
+        }
+        __syncthreads();
+        /* per element-node sums over the 5 tets at each hex node, ascending
+           tet order, into [E][8][NBC]; components 6..8 (coordinates of the
+           step before) only where a Maxwell law is compiled in, else they
+           stay zero */
+        constexpr int NC = (RSET & RS_MAXWELL) ? NBC : 6;
+        for (int o = lane; o < NC * EPW * 8; o += 64) {
+            const int rem = o / NC;                 /* element * 8 + h */
+            const int comp = o - rem * NC;
+            if (lb * EPW + (rem >> 3) >= a.E) continue;
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < 5; r++) sum += sh_nb[comp * ADJ_LDS_STRIDE + r * (EPW * 8) + rem];
+            A.nb[((size_t)lb * (EPW * 8) + rem) * NBC + comp] = sum;
+        }
     }
 }
 

@@ -278,6 +278,7 @@ extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
     return rc;
 }
 
+/* the element reverse of one step */
 static int launch_elem_adj(snh_context* c, const AdjArgs& aa) {
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
     switch (c->rset) {
@@ -305,6 +306,8 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
     HIPCHK(hipMemsetAsync(A->sbar, 0, 60 * E * sizeof(double), c->stream));
     if (A->psbar) HIPCHK(hipMemsetAsync(A->psbar, 0, 10 * E * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(A->grad, 0, E * sizeof(double), c->stream));
+    /* components 6..8 of nb are written only where a Maxwell law is compiled in */
+    HIPCHK(hipMemsetAsync(A->nb, 0, E * 8 * NBC * sizeof(double), c->stream));
     for (int which = 0; which < 2; which++) {
         const double* src = which == 0 ? coord_bar : vel_bar;
         double* dst = which == 0 ? A->xbar : A->vbar;
@@ -369,6 +372,7 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
             aa.f.xc = A->tx + (size_t)(i + 1) * 3 * N;
             aa.f.v = A->tv + (size_t)i * 3 * N;
             aa.f.stress = A->ts + (size_t)i * 60 * E;
+            aa.f.stress_out = A->ts + (size_t)(i + 1) * 60 * E;   /* sigma*_n of the replay */
             aa.f.ps = A->tp ? A->tp + (size_t)i * 10 * E : nullptr;
             aa.sbar = A->sbar;
             aa.psbar = A->psbar;
