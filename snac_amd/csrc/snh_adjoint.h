@@ -74,7 +74,9 @@ __device__ __forceinline__ int corner_slot(int t, int q) {
    coordinates (3), velocity (3), coordinates of the previous step (3) */
 #define NBC 9
 /* LDS words per component of the element-node sums in k_elem_adj:
-   5 * EPW * 8 = 240 plus padding against bank conflicts */
+   5 * EPW * 8 = 240 plus padding against bank conflicts.  The kernel's LDS
+   (20,336 B per 64-lane block) must stay within 20 KiB for 8 blocks, two
+   waves per SIMD, per CU. */
 #define ADJ_LDS_STRIDE 249
 
 struct AdjArgs {
