@@ -144,14 +144,11 @@ def adjoint_bench(make, param, steps, interval, flags=0):
     }
 
 
-def ensemble_bench(n_models, steps, graph_steps=10):
-    """Aggregate element-steps/s of an ensemble of uneven elasto-plastic
-    models: one graph launch per graph_steps steps (members on parallel
-    branches) against the same members stepped one after the other."""
+def _ensemble_times(probs, steps, graph_steps):
+    """Seconds for `steps` steps of the members in graph launches, then
+    stepped one after the other, and the autotuned element variants."""
     from snac_amd import Context
-    from snac_amd.ensemble import Ensemble, elastoplastic_ensemble
-    probs = elastoplastic_ensemble(n_models=n_models)
-    total_e = sum(p.n_elements for p in probs)
+    from snac_amd.ensemble import Ensemble
     members = [Context(p).Construct().Build().Initialise() for p in probs]
     ens = Ensemble(members, graph_steps=graph_steps, autotune=True)
     variants = [t["elem_variant"] for t in ens.tuning]
@@ -168,11 +165,33 @@ def ensemble_bench(n_models, steps, graph_steps=10):
     t_seq = time.perf_counter() - t0
     for m in members:
         m.Destroy()
+    return t_ens, t_seq, variants
+
+
+def ensemble_bench(n_models, steps, graph_steps=10):
+    """Aggregate element-steps/s of an ensemble of uneven elasto-plastic
+    models: one graph launch per graph_steps steps (members on parallel
+    branches) against the same members stepped one after the other; FP64,
+    then the same members with FP32 bulk arithmetic and FP64 state."""
+    import copy
+    from snac_amd import _abi
+    from snac_amd.ensemble import elastoplastic_ensemble
+    probs = elastoplastic_ensemble(n_models=n_models)
+    total_e = sum(p.n_elements for p in probs)
+    t_ens, t_seq, variants = _ensemble_times(probs, steps, graph_steps)
     out = {"models": n_models, "elements_total": total_e, "steps": steps, "graph_steps": graph_steps,
            "value": total_e * steps / t_ens, "unit": "element-steps/s",
            "sequential_value": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_ens,
            "elem_variants": {"tet_per_lane": variants.count(0), "element_per_lane": variants.count(1),
                              "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3)}}
+    mixed = [copy.deepcopy(p) for p in probs]
+    for p in mixed:
+        p.precision = _abi.PRECISION_MIXED
+    m_ens, m_seq, m_variants = _ensemble_times(mixed, steps, graph_steps)
+    out["mixed"] = {"value": total_e * steps / m_ens, "sequential_value": total_e * steps / m_seq,
+                    "vs_fp64": t_ens / m_ens,
+                    "elem_variants": {"tet_per_lane": m_variants.count(0),
+                                      "tet_per_wave": len(m_variants) - m_variants.count(0)}}
     out["adjoint"] = ensemble_adjoint_bench(probs, steps)
     return out
 

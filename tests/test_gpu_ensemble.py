@@ -4,7 +4,7 @@ own material and BC tables."""
 import numpy as np
 import pytest
 
-from snac_amd import Context
+from snac_amd import Context, _abi
 from snac_amd import problem as P
 from snac_amd.ensemble import Ensemble, elastoplastic_ensemble
 
@@ -46,6 +46,33 @@ def test_ensemble_matches_members_alone(schedule):
         m.Destroy()
     for p, got in zip(probs, states):
         ref = _alone(p, total)
+        for f in FIELDS:
+            assert np.array_equal(ref[f], got[f]), (p.name, f)
+
+
+def _mixed(p):
+    p.precision = _abi.PRECISION_MIXED
+    return p
+
+
+def test_mixed_precision_ensemble_matches_members_alone():
+    """Members with FP32 bulk arithmetic and FP64 state, beside an FP64
+    member, in one graph: each gives the bits it gives alone."""
+    probs = [_mixed(P.elastic_box_compression(elements=(6, 3, 6))),
+             _mixed(P.mohr_coulomb_shear(elements=(6, 4, 6))),
+             _mixed(P.maxwell_viscoelastic(elements=(5, 3, 5))),
+             P.mohr_coulomb_extension(elements=(8, 4, 6), seed=3)]
+    members = [Context(p).Construct().Build().Initialise() for p in probs]
+    ens = Ensemble(members, graph_steps=4, autotune=True)
+    for n in (8, 8, 1, 7):
+        ens.step(n)
+        ens.synchronize()
+    states = [m.state(fields=FIELDS) for m in members]
+    ens.destroy()
+    for m in members:
+        m.Destroy()
+    for p, got in zip(probs, states):
+        ref = _alone(p, 24)
         for f in FIELDS:
             assert np.array_equal(ref[f], got[f]), (p.name, f)
 
