@@ -245,7 +245,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+              "(sharding over several GPUs is designed, not built).", file=sys.stderr)
         sys.exit(2)
 
     from snac_amd import Context, _abi, metrics
