@@ -35,8 +35,11 @@
  *                                 thermal stress closed form
  * hypot                         correctly rounded (binary128), checked against
  *                                 exact rational arithmetic; glibc's is selectable
- * libm (sin, tan, pow, exp) is the system's glibc: parity unpinned at that
- * boundary beyond the checks above.
+ * libm (sin, tan, pow, exp) is the system's glibc, as in the reference's own
+ * build.  The device's versions of those four differ from glibc's by an ulp
+ * at most, so the GPU paths that call them (softened friction and dilation
+ * angles, power-law and Arrhenius viscosity) are held to the FP64 tolerance
+ * against this oracle rather than to bit equality.
  */
 #define _GNU_SOURCE
 #include "snac_oracle.h"

@@ -156,3 +156,38 @@ def test_autotune_keeps_state_and_results():
     b.Destroy()
     for f in ("coord", "velocity", "stress", "plastic_strain"):
         assert np.array_equal(sa[f], sb[f]), f
+
+
+@VARIANTS
+def test_softened_friction_and_dilation_angles(variant):
+    """Friction 30 -> 20 deg and dilation 10 -> 0 deg over plastic strain
+    0..0.05: once a tet's angles leave the table's first entry, the device's
+    sin and tan (OCML) stand in for the oracle's glibc ones, so the bar is
+    the FP64 tolerance rather than bit equality."""
+    p = P.mohr_coulomb_shear(elements=(8, 5, 6))
+    m = p.materials[0]
+    m.frictionAngle = (30.0, 20.0, 20.0)
+    m.dilationAngle = (10.0, 0.0, 0.0)
+    ref, got, rsc, gsc = run_both(p, 40, variant=variant)
+    _check(f"softened_angles[{variant}]", ref, got, rsc, gsc)
+    ps = ref["plastic_strain"]
+    assert np.sum((ps > 0.0) & (ps < 0.05)) > 100   # tets between the table's entries
+
+
+@VARIANTS
+def test_power_law_arrhenius_maxwell(variant):
+    """Non-linear Maxwell viscosity: stress exponent 1.5 and activation
+    energy 5.4e3 J/mol (the reference's default), so pow and exp (OCML on the
+    device, glibc in the oracle) enter every tet's viscosity; held to the
+    FP64 tolerance.  Under mass scaling the strain-rate invariant grows over
+    the first steps and drives the viscosity to its floor, so the run stops
+    after 10 steps, while most tets are still inside the clamp."""
+    p = P.maxwell_viscoelastic(elements=(10, 4, 10))
+    m = p.materials[0]
+    m.srexponent = 1.5
+    m.activationE = 5.4e3
+    fields = FIELDS + ("viscosity",)
+    ref, got, rsc, gsc = run_both(p, 10, fields=fields, variant=variant)
+    _check(f"power_law[{variant}]", ref, got, rsc, gsc, fields=fields)
+    v = ref["viscosity"]
+    assert np.mean((v > m.vis_min) & (v < m.vis_max)) > 0.5
