@@ -285,6 +285,13 @@ static int launch_elem_adj(snh_context* c, const AdjArgs& aa) {
     case RS_ELASTIC: hipLaunchKernelGGL(k_elem_adj<RS_ELASTIC>, grid, block, 0, c->stream, aa); break;
     case RS_PLASTIC: hipLaunchKernelGGL(k_elem_adj<RS_PLASTIC>, grid, block, 0, c->stream, aa); break;
     case RS_MAXWELL: hipLaunchKernelGGL(k_elem_adj<RS_MAXWELL>, grid, block, 0, c->stream, aa); break;
+    /* two-material layouts: a plastic or Maxwell body with elastic parts */
+    case RS_ELASTIC | RS_PLASTIC:
+        hipLaunchKernelGGL(k_elem_adj<RS_ELASTIC | RS_PLASTIC>, grid, block, 0, c->stream, aa);
+        break;
+    case RS_ELASTIC | RS_MAXWELL:
+        hipLaunchKernelGGL(k_elem_adj<RS_ELASTIC | RS_MAXWELL>, grid, block, 0, c->stream, aa);
+        break;
     default:
         hipLaunchKernelGGL(k_elem_adj<RS_ELASTIC | RS_PLASTIC | RS_MAXWELL>, grid, block, 0, c->stream, aa);
         break;
