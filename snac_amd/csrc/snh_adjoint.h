@@ -429,10 +429,12 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_adj(NodeAdjArgs A) {
  * it in two phases that share few registers: the constitutive phase holds
  * the kinematics and the stresses, the geometry phase loads the node
  * coordinates and velocities again.  Each phase fits 256 VGPRs, so the
- * kernel runs two waves per SIMD.
+ * kernel runs two waves per SIMD; the bound below holds the two-law
+ * variants there too (elastic with plastic: 256 VGPRs, no spill; all three
+ * laws: 144 B of spills per lane instead of one wave per SIMD).
  * ---------------------------------------------------------------------- */
 template <int RSET>
-__global__ void __launch_bounds__(64) k_elem_adj(AdjArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) k_elem_adj(AdjArgs A) {
     const StepArgs& a = A.f;
     __shared__ double sh_a[EPW][NTET];
     __shared__ double sh_b[EPW][NTET];
