@@ -101,6 +101,23 @@ def test_checkpoint_interval_does_not_change_gradient():
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
 
 
+def test_thousands_of_steps_with_checkpoints():
+    """2000 recorded steps of the yielding shear box (corner clamp on, which
+    keeps the frozen map's derivative bounded over long runs), checkpoints
+    every 50 (40 checkpoints, each segment replayed once) or every 125: the
+    same gradient bits, and the checkpoint count the schedule implies."""
+    p = P.mohr_coulomb_shear(elements=(5, 3, 5))
+    target = _target(p)
+    n = 2000
+    runs = [gpu_gradient(p, _abi.PARAM_COHESION, n, k, target, flags=_abi.ADJ_CORNER_CLAMP) for k in (50, 125)]
+    (_, ga, _, _, ta), (_, gb, _, _, tb) = runs
+    print(f"  2000 steps: |g|={np.linalg.norm(ga):.3e} times(50)={ta} times(125)={tb}", flush=True)
+    assert np.all(np.isfinite(ga)) and np.any(ga != 0.0) and np.linalg.norm(ga) < 1.0
+    assert np.array_equal(ga, gb)
+    assert ta["steps"] == n and ta["checkpoints"] == n // 50
+    assert tb["checkpoints"] == n // 125
+
+
 def test_recording_in_pieces_gives_the_same_gradient():
     make, param, n, interval = CASES["mc_shear"]
     p = make()
