@@ -68,6 +68,13 @@ class Context:
         self._check(self.lib.snh_timestep_zero(self.h))
         return self
 
+    def timestep_zero(self):
+        """TimeStepZero again on the current state (Context.c:978-1036):
+        after uploading a checkpoint, as the reference's restart runs it on
+        the restored fields; recomputes vmax, the length scale and dt."""
+        self._check(self.lib.snh_timestep_zero(self.h))
+        return self
+
     def Execute(self, nsteps: int):
         self.step(nsteps)
         self.synchronize()

@@ -2195,6 +2195,8 @@ static int upload_rows(snh_context* c, const double* host, double* rows, size_t 
     return SNH_OK;
 }
 
+static int clear_firewalls(snh_context* c);
+
 extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
     Range range_("snh_upload_state");
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
@@ -2227,7 +2229,7 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
     if (s->temperature)
         HIPCHK(hipMemcpy(c->temp, s->temperature, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice));
     if (s->plastic_strain && (rc = upload_rows(c, s->plastic_strain, c->ps, E, XF_TET))) return rc;
-    return SNH_OK;
+    return clear_firewalls(c);
 }
 
 static int launch_geo(snh_context* c) {
@@ -2697,14 +2699,31 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
     return SNH_OK;
 }
 
-/* the Journal_Firewall of Snac_Heat (temperature/Heat.c:121-126), raised
-   by any step since the last check */
+/* device firewall flags (non-finite velocity or dt; the thermal dt bound)
+   back to zero: done once a raised flag has been reported, and by every
+   state upload, so that a caller who restarts from a checkpoint runs on.
+   The context's stream is idle at both call sites. */
+static int clear_firewalls(snh_context* c) {
+    const int32_t zero[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync((char*)c->scal + offsetof(DevScalars, thermal_err), zero, sizeof zero,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SNH_OK;
+}
+
+/* the release-mode stress firewall (Stress.c:161-166) as a non-finite
+   velocity or dt, and the Journal_Firewall of Snac_Heat
+   (temperature/Heat.c:121-126), raised by any step since the last check;
+   each raised flag is reported once */
 static int check_firewalls(snh_context* c) {
     int32_t flags[2] = {0, 0};
     HIPCHK(hipMemcpy(flags, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof flags, hipMemcpyDeviceToHost));
+    if (!flags[0] && !flags[1]) return SNH_OK;
+    int rc;
+    if ((rc = clear_firewalls(c))) return rc;
     if (flags[1]) return set_err(SNH_ENUMERIC, "timeStep %u: a velocity or dt is not finite (a NaN or Inf "
                                                "reached the state; Stress.c:161-166)", c->timeStep);
-    if (c->thermal && flags[0])
+    if (c->thermal)
         return set_err(SNH_ENUMERIC, "dt_thermal / dt_mech <= 1: check lambda, mu, density, minLengthScale, "
                                      "thermal_conduct, heatCapacity (temperature/Heat.c:121-126)");
     return SNH_OK;

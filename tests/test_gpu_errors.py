@@ -49,11 +49,18 @@ def test_nan_in_the_state_stops_the_run():
     c = Context(p).Construct().Build().Initialise()
     try:
         c.Execute(3)
-        st = c.state(fields=("stress",))["stress"]
+        good = c.state(fields=("coord", "velocity", "stress"))
+        st = good["stress"].copy()
         st[7, 3, 1] = np.nan
         c.set_state(stress=st)
         c.step(2)
         _raises(c.synchronize, _abi.SNH_ENUMERIC, "not finite")
+        # reported once; a restart from the good state runs on
+        c.set_state(**good)
+        c.timestep_zero()
+        c.step(2)
+        c.synchronize()
+        assert np.all(np.isfinite(c.state(fields=("stress",))["stress"]))
     finally:
         c.Destroy()
 
