@@ -185,7 +185,9 @@ int snh_set_element_field(snh_context* ctx, int field, const double* values, siz
 
 /* Copy state in/out (host memory).  Upload also re-applies nothing: the
  * caller passes the state as the reference holds it after
- * _Snac_Context_InitialConditions (Context.c:742-774). */
+ * _Snac_Context_InitialConditions (Context.c:742-774).  An upload clears the
+ * device firewall flags; a restart is an upload followed by
+ * snh_timestep_zero. */
 int snh_upload_state(snh_context* ctx, const snh_state* state);
 int snh_download_state(snh_context* ctx, snh_state* state);
 
@@ -269,6 +271,10 @@ int snh_sync(snh_context* ctx);
 int snh_step(snh_context* ctx, int nsteps);
 
 int snh_get_scalars(snh_context* ctx, snh_scalars* out);
+/* Waits for the context's stream, then reports a device firewall raised by
+ * any step since the last check (SNH_ENUMERIC: a non-finite velocity or dt,
+ * the stress firewall of Stress.c:161-166; or the thermal dt bound of
+ * temperature/Heat.c:121-126), once. */
 int snh_synchronize(snh_context* ctx);
 
 /* Per-kernel durations measured with HIP events recorded on the context's
