@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores given to this job")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--topo-kappa", type=float, default=0.0,
+                    help="top-surface diffusivity (the reference's default input uses 5e-6; 0 = off)")
     ap.add_argument("--elements", default="", help="override the mesh, e.g. 256x64x128")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--adjoint-steps", type=int, default=500)
@@ -251,6 +253,7 @@ def main():
     from snac_amd import Context, _abi, metrics
     problem = make_problem(args.workload, args.elements)
     problem.precision = _abi.PRECISION_MIXED if args.precision == "mixed" else _abi.PRECISION_FP64
+    problem.topo_kappa = args.topo_kappa
     E = problem.n_elements
 
     ctx = Context(problem).Construct().Build().Initialise()
@@ -313,7 +316,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": problem.name, "elements": list(problem.elements),
-                   "nodes": list(problem.nodes), "precision": args.precision},
+                   "nodes": list(problem.nodes), "precision": args.precision, "topo_kappa": args.topo_kappa},
         "roofline": {
             "bound": "hbm",
             "kernel": "k_" + dom,

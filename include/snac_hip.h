@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SNH_ABI_VERSION 2
+#define SNH_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------- */
 #define SNH_OK            0
@@ -121,7 +121,12 @@ typedef struct snh_config {
     double   gravity;             /* acts on -y (Snac/libSnac/src/Force.c:173)  */
     double   demf;                /* local damping factor                       */
     double   strain_inert;        /* dynamic mass-scaling factor                */
-    double   topo_kappa;          /* must be 0: surface diffusion is not on the path */
+    double   topo_kappa;          /* top-surface diffusivity (0 = off; the
+                                     reference's default is 5e-6), Cartesian
+                                     meshes more than two nodes wide in x and z */
+    double   topoGradCriterion;   /* diffusion runs once the running maximum of
+                                     |grad y| over the top surface reaches it
+                                     (the reference's default 1e-16)        */
     uint32_t nmaterials;
     const snh_material* materials;
     uint32_t nbcs;
@@ -158,6 +163,8 @@ typedef struct snh_scalars {
     double   minLengthScale;
     double   vmax;                /* max |v| over constrained nodes, floored    */
     uint32_t timeStep;            /* steps completed                           */
+    double   topoGradMax;         /* running max of |grad y| on the top surface
+                                     (topography diffusion; 0 when off)     */
 } snh_scalars;
 
 typedef struct snh_context snh_context;

@@ -188,7 +188,7 @@ class Oracle:
         s = _abi.Scalars()
         _check(lib().orc_get_scalars(self.h, C.byref(s)))
         return {"dt": s.dt, "speedOfSound": s.speedOfSound, "minLengthScale": s.minLengthScale,
-                "vmax": s.vmax, "timeStep": s.timeStep}
+                "vmax": s.vmax, "timeStep": s.timeStep, "topoGradMax": s.topoGradMax}
 
     def state(self):
         coord = np.zeros((self.N, 3))

@@ -140,6 +140,8 @@ struct orc_context {
     /* context scalars */
     double dt, speedOfSound, minLengthScale, gravity, demf, strain_inert;
     double vmax;
+    double topoGradMax;            /* running maximum of |grad y| on the top
+                                      surface (Context.c:759-764), never reset */
     unsigned timeStep;
     int zero_done;
     /* optional record of the discrete decisions of each step (test
@@ -298,8 +300,8 @@ orc_context* orc_create(const snh_config* cfg) {
         fail("orc_create: bad material list");
         return NULL;
     }
-    if (cfg->topo_kappa > 0) {
-        fail("orc_create: topo_kappa > 0 (surface diffusion) is not on the restated path");
+    if (cfg->topo_kappa > 0 && (cfg->meshSizeI < 3 || cfg->meshSizeK < 3)) {
+        fail("orc_create: topo_kappa > 0 on a mesh two nodes wide (diffTopo_alongX/Z) is not restated");
         return NULL;
     }
     orc_context* c = (orc_context*)calloc(1, sizeof *c);
@@ -454,6 +456,7 @@ int orc_get_scalars(orc_context* c, snh_scalars* out) {
     out->minLengthScale = c->minLengthScale;
     out->vmax = c->vmax;
     out->timeStep = c->timeStep;
+    out->topoGradMax = c->topoGradMax;
     return SNH_OK;
 }
 
@@ -1275,13 +1278,92 @@ int orc_update_node_preprocess(orc_context* c, uint32_t n, double inertialMass, 
     return SNH_OK;
 }
 
-/* Snac_UpdateNodeMomentum, UpdateNode.c:60-72 (topo_kappa == 0) */
+/* ---------------------------------------------------------------------------
+ * Topography diffusion of the top surface (topo_kappa > 0), Cartesian meshes
+ * more than two nodes wide in x and z.  The reference runs it per node inside
+ * the second node pass, in node order, after the node's coordinate update:
+ * neighbours with a smaller index already hold their new coordinates (and
+ * diffused heights), those with a larger one still hold the old ones.  The
+ * neighbour slots of IJK6Topology_BuildNeighbours (StGermain/Discretisation/
+ * Geometry/src/IJK6Topology.c:283-343, fixed positions): 0 +x, 2 +z, 3 -x,
+ * 5 -z.
+ * ------------------------------------------------------------------------- */
+static int top_interior(const orc_context* c, unsigned i, unsigned k) {
+    return i > 0 && i < c->I - 1 && k > 0 && k < c->K - 1;
+}
+
+/* getMaxTopoGrad, UpdateNode.c:161-260 (Cartesian branch) */
+static void topo_max_grad(orc_context* c, unsigned n) {
+    const unsigned i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
+    if (c->timeStep <= 1) return;
+    if (j < c->J - 1) return;
+    if (!top_interior(c, i, k)) return;
+    const double* X0 = c->coord[n];
+    const double* xR = c->coord[n + 1];
+    const double* xF = c->coord[n + c->I * c->J];
+    const double dydx = (xR[1] - X0[1]) / (xR[0] - X0[0]);
+    const double dydz = (xF[1] - X0[1]) / (xF[2] - X0[2]);
+    const double g = sqrt(dydx * dydx + dydz * dydz);
+    c->topoGradMax = c->topoGradMax > g ? c->topoGradMax : g;
+}
+
+/* Snac_UpdateNodeMomentum_DiffTopo and _UpdateInteriorTopo,
+   UpdateNode.c:359-505,647-714 (Cartesian branch): dh = kappa dt lapl(y)
+   from the five-point stencil, added to the node's height */
+static void topo_diffuse(orc_context* c, unsigned n) {
+    const unsigned i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
+    if (c->timeStep <= 1) return;
+    if (j < c->J - 1 || c->topoGradMax < c->cfg.topoGradCriterion) return;
+    if (!top_interior(c, i, k)) return;
+    const unsigned IJ = c->I * c->J;
+    double* X0 = c->coord[n];
+    const double* xR = c->coord[n + 1];
+    const double* xF = c->coord[n + IJ];
+    const double* xL = c->coord[n - 1];
+    const double* xB = c->coord[n - IJ];
+    const double dydx1 = (xR[1] - X0[1]) / (xR[0] - X0[0]);
+    const double dydx2 = (X0[1] - xL[1]) / (X0[0] - xL[0]);
+    const double d2ydx2 = (dydx1 - dydx2) / (0.5 * (xR[0] - xL[0]));
+    const double dydz1 = (xF[1] - X0[1]) / (xF[2] - X0[2]);
+    const double dydz2 = (X0[1] - xB[1]) / (X0[2] - xB[2]);
+    const double d2ydz2 = (dydz1 - dydz2) / (0.5 * (xF[2] - xB[2]));
+    const double lapl_topo = d2ydx2 + d2ydz2;
+    const double dh = c->cfg.topo_kappa * c->dt * lapl_topo;
+    X0[1] += dh;
+}
+
+/* Snac_UpdateNodeMomentum_AdjustEdgeTopo, UpdateNode.c:716-857 (Cartesian
+   branch): an edge node takes the height of its inner neighbour, a corner the
+   mean of its two */
+static void topo_adjust_edge(orc_context* c, unsigned n) {
+    const unsigned i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
+    if (c->timeStep <= 1) return;
+    if (j < c->J - 1 || c->topoGradMax < c->cfg.topoGradCriterion) return;
+    const unsigned IJ = c->I * c->J;
+    const double *xN1 = NULL, *xN2 = NULL;
+    if (i == 0) xN1 = c->coord[n + 1];
+    else if (i == c->I - 1) xN1 = c->coord[n - 1];
+    if (k == 0) xN2 = c->coord[n + IJ];
+    else if (k == c->K - 1) xN2 = c->coord[n - IJ];
+    double* X0 = c->coord[n];
+    if (xN1 && !xN2) X0[1] = xN1[1];
+    else if (!xN1 && xN2) X0[1] = xN2[1];
+    else if (xN1 && xN2) X0[1] = 0.5 * (xN1[1] + xN2[1]);
+}
+
+/* Snac_UpdateNodeMomentum, UpdateNode.c:60-72, and the topography hooks
+   appended after it on Snac_EP_UpdateNodeMomentum (Context.c:578-592) */
 int orc_update_node(orc_context* c, uint32_t n) {
     double* x = c->coord[n];
     const double* v = c->node[n].velocity;
     x[0] += c->dt * v[0];
     x[1] += c->dt * v[1];
     x[2] += c->dt * v[2];
+    if (c->cfg.topo_kappa > 0) {
+        topo_max_grad(c, n);
+        topo_diffuse(c, n);
+        topo_adjust_edge(c, n);
+    }
     return SNH_OK;
 }
 
@@ -1519,6 +1601,14 @@ int orc_loop_nodes(orc_context* c) {
     for (long n = 0; n < N; n++) {
         double mass, balance[3];
         orc_force(c, (unsigned)n, c->speedOfSound, &mass, &c->node[n].inertialMass, c->node[n].force, balance);
+    }
+    if (c->cfg.topo_kappa > 0) {
+        /* the topography hooks read neighbours in node order: sequential */
+        for (long n = 0; n < N; n++) {
+            orc_update_node_preprocess(c, (unsigned)n, c->node[n].inertialMass, c->node[n].force);
+            orc_update_node(c, (unsigned)n);
+        }
+        return SNH_OK;
     }
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (long n = 0; n < N; n++) {

@@ -88,6 +88,7 @@ class Config(C.Structure):
         ("demf", C.c_double),
         ("strain_inert", C.c_double),
         ("topo_kappa", C.c_double),
+        ("topoGradCriterion", C.c_double),
         ("nmaterials", C.c_uint32),
         ("materials", C.POINTER(Material)),
         ("nbcs", C.c_uint32),
@@ -128,6 +129,7 @@ class Scalars(C.Structure):
         ("minLengthScale", C.c_double),
         ("vmax", C.c_double),
         ("timeStep", C.c_uint32),
+        ("topoGradMax", C.c_double),
     ]
 
 
@@ -217,7 +219,7 @@ SIGNATURES = {
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 
-ABI_VERSION = 2          # SNH_ABI_VERSION of include/snac_hip.h
+ABI_VERSION = 3          # SNH_ABI_VERSION of include/snac_hip.h
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libsnac_hip.so")

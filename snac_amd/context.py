@@ -219,7 +219,7 @@ class Context:
         s = _abi.Scalars()
         self._check(self.lib.snh_get_scalars(self.h, C.byref(s)))
         return {"dt": s.dt, "speedOfSound": s.speedOfSound, "minLengthScale": s.minLengthScale,
-                "vmax": s.vmax, "timeStep": s.timeStep}
+                "vmax": s.vmax, "timeStep": s.timeStep, "topoGradMax": s.topoGradMax}
 
     def state(self, fields=None) -> Dict[str, np.ndarray]:
         """Download fields (default: coord, velocity, stress, plastic_strain,

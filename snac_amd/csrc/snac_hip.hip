@@ -40,6 +40,7 @@ using namespace snh;
 
 #define ELEM_BLOCK 128
 #define NODE_BLOCK 128
+#define TOPO_MAX_I 4096   /* nodes along x the topography sweep holds in LDS */
 #define GEO_BLOCK 256
 #define FIN_BLOCK 1024
 
@@ -1330,6 +1331,211 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
     if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
 }
 
+/* --------------------------------------------------------------------------
+ * Topography diffusion of the top surface (topo_kappa > 0; UpdateNode.c:
+ * 60-72,161-260,359-505,647-857, Cartesian branches).  The reference runs it
+ * per node inside the second node pass, in node order, right after the
+ * node's coordinate update: getMaxTopoGrad raises the running maximum of
+ * |grad y| (never reset, Context.c:759-764), and once that reaches
+ * topoGradCriterion, DiffTopo adds kappa dt lapl(y) to a top interior node
+ * and AdjustEdgeTopo gives an edge node the height of its inner neighbour
+ * (the mean of two at a corner).  A top node reads the final heights of its
+ * -x and -z neighbours and the old coordinates of its +x and +z ones (slots
+ * 3, 5 and 0, 2 of IJK6Topology_BuildNeighbours), so the final height of
+ * node (i, k) depends on (i-1, k) and (i, k-1) only: the surface is swept
+ * by anti-diagonals d = i + k.
+ *
+ * k_topo_prep (one thread per top node) computes everything the sweep does
+ * not change -- the node's gradient for the running maximum, the
+ * differences with its old +x/+z neighbours and the stencil's denominators
+ * with their correctly rounded reciprocals -- into records stored in
+ * diagonal order, and reduces the maximum and the first node (in node
+ * order) whose gradient reaches the criterion per block.  k_topo (one
+ * block) finishes the reductions and sweeps the diagonals with the heights
+ * of diagonal d-1 in LDS, prefetching records several diagonals ahead
+ * (they come from other XCDs' writes, about 2 us away); a
+ * division whose divisor the sweep does not change is done as
+ * snh_div_by with the stored reciprocal (correctly rounded, as the
+ * reference's).  Both run after k_node with its arguments: xc the old
+ * coordinates, xn the updated ones, whose heights k_topo finalises.
+ * ------------------------------------------------------------------------ */
+#define TOPO_REC 11          /* doubles per top-node record            */
+#define TOPO_PREP_BLOCK 256
+
+struct TopoArgs {
+    double* rec;             /* [TOPO_REC][I K] records, diagonal order */
+    double* pmax;            /* [nparts] block maxima of |grad y|       */
+    int* pfirst;             /* [nparts] block minima of the first node */
+    int nparts;
+    double kappa, crit;
+};
+
+__device__ __forceinline__ int topo_ilo(int d, int K) { return d - (K - 1) > 0 ? d - (K - 1) : 0; }
+/* first record of diagonal d: the top nodes with i + k < d on an I x K
+   surface, sum over s < d of (min(s, I-1) + 1) - max(0, s - K + 1) */
+__host__ __device__ __forceinline__ long long topo_off(int d, int I, int K) {
+    const long long dd = d, II = I, KK = K;
+    const long long a = dd <= II ? dd * (dd + 1) / 2 : II * (II + 1) / 2 + (dd - II) * II;
+    const long long b = dd > KK ? (dd - KK) * (dd - KK + 1) / 2 : 0;
+    return a - b;
+}
+
+__global__ void __launch_bounds__(TOPO_PREP_BLOCK) k_topo_prep(StepArgs a, TopoArgs t) {
+    __shared__ double smax[TOPO_PREP_BLOCK];
+    __shared__ int sfirst[TOPO_PREP_BLOCK];
+    const int I = a.I, J = a.J, K = a.K, IJ = I * J, P = I * K;
+    const int p = blockIdx.x * TOPO_PREP_BLOCK + threadIdx.x;   /* k I + i: node order */
+    double g = 0.0;
+    int first = P;
+    if (p < P) {
+        const int k = p / I, i = p - k * I;
+        const int n = I * (J - 1) + i + IJ * k;
+        const int d = i + k;
+        const size_t N = (size_t)a.N, PP = (size_t)P;
+        const size_t pos = (size_t)topo_off(d, I, K) + (size_t)(i - topo_ilo(d, K));
+        double r[TOPO_REC];
+#pragma unroll
+        for (int q = 0; q < TOPO_REC; q++) r[q] = 0.0;
+        r[0] = a.xn[N + n];
+        if (i > 0 && i < I - 1 && k > 0 && k < K - 1) {
+            const double x0 = a.xn[n], y0 = r[0], z0 = a.xn[2 * N + n];
+            const double xR = a.xc[n + 1], yR = a.xc[N + n + 1];
+            const double yF = a.xc[N + n + IJ], zF = a.xc[2 * N + n + IJ];
+            const double xL = a.xn[n - 1], zB = a.xn[2 * N + n - IJ];
+            const double dydx1 = (yR - y0) / (xR - x0);
+            const double dydz1 = (yF - y0) / (zF - z0);
+            r[1] = dydx1;
+            r[2] = x0 - xL;
+            r[3] = 1.0 / r[2];
+            r[4] = 0.5 * (xR - xL);
+            r[5] = 1.0 / r[4];
+            r[6] = dydz1;
+            r[7] = z0 - zB;
+            r[8] = 1.0 / r[7];
+            r[9] = 0.5 * (zF - zB);
+            r[10] = 1.0 / r[9];
+            /* getMaxTopoGrad's dydx, dydz are the same quotients */
+            g = sqrt(dydx1 * dydx1 + dydz1 * dydz1);
+            if (!(g < t.crit)) first = p;
+        } else {
+            if (i == 0) r[1] = a.xc[N + n + 1];    /* old height of (1, k)  */
+            if (k == 0) r[6] = a.xc[N + n + IJ];   /* old height of (i, 1)  */
+        }
+#pragma unroll
+        for (int q = 0; q < TOPO_REC; q++) t.rec[q * PP + pos] = r[q];
+    }
+    smax[threadIdx.x] = g;
+    sfirst[threadIdx.x] = first;
+    __syncthreads();
+    for (int w = TOPO_PREP_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = smax[threadIdx.x + w];
+            smax[threadIdx.x] = smax[threadIdx.x] > o ? smax[threadIdx.x] : o;
+            const int of = sfirst[threadIdx.x + w];
+            if (of < sfirst[threadIdx.x]) sfirst[threadIdx.x] = of;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        t.pmax[blockIdx.x] = smax[0];
+        t.pfirst[blockIdx.x] = sfirst[0];
+    }
+}
+
+/* BLOCK: the most threads the sweep may be launched with (a diagonal's
+   nodes, at most min(I, K)); PF: diagonals prefetched ahead -- deeper with
+   the smaller block, whose waves may hold more registers */
+template <int BLOCK, int PF>
+__global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
+    constexpr int TOPO_PF = PF;
+    __shared__ double yd[2][TOPO_MAX_I];
+    __shared__ double rmax[BLOCK];
+    __shared__ int rfirst[BLOCK];
+    const int I = a.I, J = a.J, K = a.K, IJ = I * J, P = I * K;
+    const int tid = threadIdx.x, B = blockDim.x;
+    const size_t N = (size_t)a.N, PP = (size_t)P;
+    const double prev = a.scal->topoGradMax;
+    const double kd = t.kappa * a.scal->dt;
+
+    /* the running maximum: reaches the criterion at node pfirst (node order) */
+    double m = 0.0;
+    int f = P;
+    for (int q = tid; q < t.nparts; q += B) {
+        m = m > t.pmax[q] ? m : t.pmax[q];
+        if (t.pfirst[q] < f) f = t.pfirst[q];
+    }
+    rmax[tid] = m;
+    rfirst[tid] = f;
+    __syncthreads();
+    for (int w = B / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            rmax[tid] = rmax[tid] > rmax[tid + w] ? rmax[tid] : rmax[tid + w];
+            if (rfirst[tid + w] < rfirst[tid]) rfirst[tid] = rfirst[tid + w];
+        }
+        __syncthreads();
+    }
+    const int pfirst = (prev < t.crit) ? rfirst[0] : 0;
+    if (tid == 0) a.scal->topoGradMax = prev > rmax[0] ? prev : rmax[0];
+
+    /* the sweep; thread tid holds node i = ilo(d) + tid of diagonal d.  The
+       records of diagonal d + TOPO_PF are loaded while diagonal d is
+       computed, always from a valid position (clamped), and the node's
+       update is computed without branches (interior and edge formulas both
+       evaluated, the right one selected), so that nothing waits on the
+       loads before their use three barriers later. */
+    const int D = I + K - 1;
+    double ring[TOPO_PF][TOPO_REC];
+#pragma unroll
+    for (int r = 0; r < TOPO_PF; r++) {
+        const int dd = r < D ? r : D - 1;
+        const int cn = (dd < I - 1 ? dd : I - 1) - topo_ilo(dd, K) + 1;
+        const size_t pos = (size_t)topo_off(dd, I, K) + (tid < cn ? tid : cn - 1);
+#pragma unroll
+        for (int q = 0; q < TOPO_REC; q++) ring[r][q] = t.rec[q * PP + pos];
+    }
+    for (int d0 = 0; d0 < D; d0 += TOPO_PF) {
+#pragma unroll
+        for (int r = 0; r < TOPO_PF; r++) {
+            const int d = d0 + r;
+            if (d >= D) break;
+            const int ilo = topo_ilo(d, K);
+            const int cnt = (d < I - 1 ? d : I - 1) - ilo + 1;
+            const bool valid = tid < cnt;
+            const int i = valid ? ilo + tid : ilo, k = d - i;
+            const double* yprev = yd[(d + 1) & 1];   /* final heights of diagonal d-1, by i */
+            const double yl = yprev[i > 0 ? i - 1 : 0];
+            const double yb = yprev[i];
+            const double* rc = ring[r];
+            const double y0 = rc[0];
+            /* DiffTopo + UpdateInteriorTopo */
+            const double dydx2 = snh_div_by(y0 - yl, rc[2], rc[3]);
+            const double d2ydx2 = snh_div_by(rc[1] - dydx2, rc[4], rc[5]);
+            const double dydz2 = snh_div_by(y0 - yb, rc[7], rc[8]);
+            const double d2ydz2 = snh_div_by(rc[6] - dydz2, rc[9], rc[10]);
+            const double lapl_topo = d2ydx2 + d2ydz2;
+            const double yi = y0 + kd * lapl_topo;
+            /* AdjustEdgeTopo: xN1 along x, xN2 along z, the old height of a
+               neighbour not yet updated (+x, +z), the final one otherwise */
+            const bool h1 = (i == 0 || i == I - 1), h2 = (k == 0 || k == K - 1);
+            const double y1 = (i == 0) ? rc[1] : yl;
+            const double y2 = (k == 0) ? rc[6] : yb;
+            const double ye = (h1 && !h2) ? y1 : ((!h1 && h2) ? y2 : 0.5 * (y1 + y2));
+            const bool active = valid && k * I + i >= pfirst;
+            const double y = active ? ((h1 || h2) ? ye : yi) : y0;
+            if (valid) yd[d & 1][i] = y;
+            if (active) a.xn[N + I * (J - 1) + i + IJ * k] = y;
+            {
+                const int dn = d + TOPO_PF < D ? d + TOPO_PF : D - 1;
+                const int cn = (dn < I - 1 ? dn : I - 1) - topo_ilo(dn, K) + 1;
+                const size_t pos = (size_t)topo_off(dn, I, K) + (tid < cn ? tid : cn - 1);
+#pragma unroll
+                for (int q = 0; q < TOPO_REC; q++) ring[r][q] = t.rec[q * PP + pos];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 /* |v| over constrained nodes without an update (TimeStepZero's Sync) */
 __global__ void __launch_bounds__(NODE_BLOCK) k_vmax(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
@@ -1711,6 +1917,13 @@ struct snh_context {
     DevBC* dbc;          /* velocity-condition table [256], entry 0 unused */
     double* tmp;    /* download staging [6][10][E] */
     double* stage;  /* host-order records of one field, max(60 E, 3 N) doubles */
+    /* topography diffusion (topo_kappa > 0): records and partial reductions
+       of k_topo_prep / k_topo */
+    double* topo_rec;
+    double* topo_pmax;
+    int* topo_pfirst;
+    int topo_parts;
+    int topo_threads;   /* sweep block: a power of two >= min(I, K) */
     int n_geo_blocks, n_node_blocks;
     unsigned rset;  /* rheologies present */
     bool zero_done;
@@ -1890,7 +2103,8 @@ static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
                     c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
                     c->dmat, c->dbc, c->diag_buf, c->force_buf, c->temp, c->temp0, c->tpre, c->hflux,
-                    c->tstress, c->dens, c->tbc, c->tbcv, c->stage};
+                    c->tstress, c->dens, c->tbc, c->tbcv, c->stage, c->topo_rec, c->topo_pmax,
+                    c->topo_pfirst};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1902,8 +2116,16 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         return set_err(SNH_EINVAL, "mesh needs at least 2 nodes per axis");
     if (cfg->nmaterials < 1 || cfg->nmaterials > SNH_MAX_MATERIALS || !cfg->materials)
         return set_err(SNH_EINVAL, "need 1..%d materials", SNH_MAX_MATERIALS);
-    if (cfg->topo_kappa > 0)
-        return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0: surface diffusion is order dependent and not on the path");
+    if (cfg->topo_kappa > 0) {
+        if (cfg->meshSizeI < 3 || cfg->meshSizeK < 3)
+            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 on a mesh two nodes wide (diffTopo_alongX/Z)");
+        if (cfg->meshSizeI > TOPO_MAX_I)
+            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than %d nodes along x", TOPO_MAX_I);
+        if (cfg->meshSizeI > 1024 && cfg->meshSizeK > 1024)
+            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than 1024 nodes along both x and z");
+        if (cfg->precision != SNH_PRECISION_FP64)
+            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 runs on the FP64 path");
+    }
     if (cfg->dtType == SNH_DT_WAVE) return set_err(SNH_EUNSUPPORTED, "dtType wave is not implemented on the device");
     if (cfg->dtType != SNH_DT_DYNAMIC && cfg->dtType != SNH_DT_CONSTANT) return set_err(SNH_EINVAL, "bad dtType");
     if (cfg->precision != SNH_PRECISION_FP64 && cfg->precision != SNH_PRECISION_MIXED)
@@ -1985,12 +2207,24 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     ALLOC(c->scal, sizeof(DevScalars));
     ALLOC(c->dmat, SNH_MAX_MATERIALS * sizeof(DevMaterial));
     ALLOC(c->dbc, 256 * sizeof(DevBC));
+    if (cfg->topo_kappa > 0) {
+        const size_t P = (size_t)c->I * c->K;
+        c->topo_parts = (int)((P + TOPO_PREP_BLOCK - 1) / TOPO_PREP_BLOCK);
+        ALLOC(c->topo_rec, TOPO_REC * P * sizeof(double));
+        ALLOC(c->topo_pmax, (size_t)c->topo_parts * sizeof(double));
+        ALLOC(c->topo_pfirst, (size_t)c->topo_parts * sizeof(int));
+    }
 #undef ALLOC
     if (!ok) {
         free_all(c);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return set_err(SNH_ENOMEM, "device allocation failed (%zu nodes, %zu elements)", N, E);
+    }
+    if (cfg->topo_kappa > 0) {
+        const int w = c->I < c->K ? c->I : c->K;
+        c->topo_threads = 64;
+        while (c->topo_threads < w) c->topo_threads *= 2;
     }
 
     /* initial mesh (BlockGeometry.c:326-349), zero velocity + BCs, zero stress */
@@ -2069,6 +2303,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         s0.vmax = 0.0;
         s0.thermal_err = 0;
         s0.nonfinite = 0;
+        s0.topoGradMax = 0.0;
         if (hipMemcpy(c->scal, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_err(SNH_EHIP, "scalar upload failed");
             break;
@@ -2337,7 +2572,8 @@ static int sched_index(const snh_context* c, uint32_t t) {
     return (int)(t - c->sched_base);
 }
 
-static int launch_node(snh_context* c) {
+/* refStep: the reference's timeStep of the step being run (1 for the first) */
+static int launch_node(snh_context* c, int refStep) {
     KTimer kt(c, KK_NODE);
     StepArgs a = make_args(c);
     const int si = sched_index(c, c->timeStep);
@@ -2353,6 +2589,23 @@ static int launch_node(snh_context* c) {
     else
         hipLaunchKernelGGL((k_node<false, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
+    if (c->cfg.topo_kappa > 0 && refStep > 1) {
+        /* before the buffer flip: xc still holds the old coordinates */
+        TopoArgs t;
+        t.rec = c->topo_rec;
+        t.pmax = c->topo_pmax;
+        t.pfirst = c->topo_pfirst;
+        t.nparts = c->topo_parts;
+        t.kappa = c->cfg.topo_kappa;
+        t.crit = c->cfg.topoGradCriterion;
+        hipLaunchKernelGGL(k_topo_prep, dim3(c->topo_parts), dim3(TOPO_PREP_BLOCK), 0, c->stream, a, t);
+        HIPCHK(hipGetLastError());
+        if (c->topo_threads <= 256)
+            hipLaunchKernelGGL((k_topo<256, 8>), dim3(1), dim3(c->topo_threads), 0, c->stream, a, t);
+        else
+            hipLaunchKernelGGL((k_topo<1024, 3>), dim3(1), dim3(c->topo_threads), 0, c->stream, a, t);
+        HIPCHK(hipGetLastError());
+    }
     c->cur ^= 1;
     return SNH_OK;
 }
@@ -2584,7 +2837,7 @@ extern "C" int snh_loop_nodes_momentum(snh_context* c) {
     Range range_("snh_loop_nodes_momentum");
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_loop_nodes_momentum before snh_timestep_zero");
-    return launch_node(c);
+    return launch_node(c, (int)c->timeStep);   /* snh_calc_stresses counted this step */
 }
 
 extern "C" int snh_loop_elements_momentum(snh_context* c) {
@@ -2621,7 +2874,7 @@ extern "C" int snh_step(snh_context* c, int nsteps) {
             if ((rc = launch_heat(c, false))) return rc;
         }
         if ((rc = launch_elem(c))) return rc;
-        if ((rc = launch_node(c))) return rc;
+        if ((rc = launch_node(c, (int)c->timeStep + 1))) return rc;
         if ((rc = launch_geo(c))) return rc;
         if ((rc = launch_fin(c))) return rc;
         c->timeStep++;
@@ -2745,6 +2998,7 @@ extern "C" int snh_get_scalars(snh_context* c, snh_scalars* out) {
     out->minLengthScale = s.minLen;
     out->vmax = s.vmax;
     out->timeStep = c->timeStep;
+    out->topoGradMax = s.topoGradMax;
     return SNH_OK;
 }
 
@@ -3010,6 +3264,8 @@ extern "C" int snh_ensemble_create(snh_context* const* members, int32_t n, int32
         if (c->sched_n > 0 || c->profiling || c->adj)
             return set_err(SNH_ESTATE, "ensemble member %d has a schedule, profiling or an adjoint", i);
         if (c->thermal) return set_err(SNH_EUNSUPPORTED, "ensemble member %d runs the temperature plugin", i);
+        if (c->cfg.topo_kappa > 0)
+            return set_err(SNH_EUNSUPPORTED, "ensemble member %d diffuses its topography (topo_kappa > 0)", i);
         for (int j = 0; j < i; j++)
             if (members[j] == c || members[j]->stream == c->stream)
                 return set_err(SNH_EINVAL, "ensemble members need distinct contexts and streams");

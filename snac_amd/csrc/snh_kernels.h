@@ -74,6 +74,8 @@ struct DevScalars {
     double vmax;
     int32_t thermal_err;   /* set when dt_thermal / dt_mech <= 1 (Heat.c:121-126) */
     int32_t nonfinite;     /* set when a velocity or dt stops being finite */
+    double topoGradMax;    /* running max of |grad y| on the top surface
+                              (Context.c:759-764: never reset)            */
 };
 
 struct DevBC {

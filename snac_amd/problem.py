@@ -131,6 +131,7 @@ class Problem:
     demf: float = f32(0.8)
     strain_inert: float = f32(1.0e-5)
     topo_kappa: float = 0.0
+    topoGradCriterion: float = 1.0e-16
     precision: int = _abi.PRECISION_FP64
     element_material: Optional[np.ndarray] = None        # uint8 [E]
     fields: Dict[int, np.ndarray] = field(default_factory=dict)   # SNH_FIELD_* -> [E]
@@ -173,6 +174,7 @@ class Problem:
         cfg.demf = self.demf
         cfg.strain_inert = self.strain_inert
         cfg.topo_kappa = self.topo_kappa
+        cfg.topoGradCriterion = self.topoGradCriterion
         cfg.nmaterials = len(self.materials)
         cfg.materials = C.cast(mats, C.POINTER(_abi.Material))
         cfg.nbcs = len(self.bcs)
