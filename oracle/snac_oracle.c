@@ -21,6 +21,11 @@
  *                                 (vx = 0.00834447 after 10 iterations) and
  *                                 -coord (0.550852, 1.1)
  *   dt (Sync)                     test-basic timeStep.0 (1.80422e+07)
+ *   topography diffusion          parabolic top surface at rest: the
+ *                                 five-point stencil's node-order recurrence
+ *                                 dh_i = kappa dt (2a + dh_(i-1)/h^2) along x
+ *                                 and z, the switch-on node, the edge copies
+ *                                 and the running maximum of |grad y|
  *   elastic                       uniaxial strain closed form
  *   plastic_BI eigen-solver       numpy.linalg.eigh on random symmetric tensors,
  *                                 int-abs truncation cases
