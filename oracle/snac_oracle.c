@@ -1712,6 +1712,7 @@ int orc_hydrostatic_ic(orc_context* c) {
 
 /* Snac_Context_TimeStepZero, Context.c:978-1036 */
 int orc_timestep_zero(orc_context* c) {
+    c->topoGradMax = 0.0;   /* a run begins with it at zero (Context.c:759-764) */
     for (unsigned e = 0; e < c->E; e++) {
         orc_element* el = &c->elem[e];
         const snh_material* m = &c->mat[el->material_I];

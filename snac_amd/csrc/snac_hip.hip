@@ -2809,6 +2809,11 @@ extern "C" int snh_timestep_zero(snh_context* c) {
     Range range_("snh_timestep_zero");
     if (!c) return set_err(SNH_EINVAL, "null context");
     int rc;
+    /* a run (or a restart, a new run of the reference) begins with the
+       running topography gradient at zero (Context.c:759-764) */
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync((char*)c->scal + offsetof(DevScalars, topoGradMax), &zero, sizeof zero,
+                          hipMemcpyHostToDevice, c->stream));
     StepArgs a = make_args(c);
     hipLaunchKernelGGL(k_vmax, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
