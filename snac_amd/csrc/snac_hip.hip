@@ -1837,6 +1837,9 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvm
         if (a.dtType == SNH_DT_DYNAMIC) {
             s->dt = mn * F_045 * a.strain_inert / vmax;
             s->speedOfSound = mn * 0.5 / s->dt;
+        } else if (a.dtType == SNH_DT_WAVE) {   /* Context.c:1255-1259 */
+            s->speedOfSound = a.c_wave;
+            s->dt = 0.5 * mn / s->speedOfSound;
         }
         if (!isfinite(s->dt)) s->nonfinite = 1;
     }
@@ -1930,6 +1933,11 @@ struct snh_context {
     bool rotation_pending;
     bool needs_plastic;
     bool needs_visc;
+    /* Vp = sqrt((lambda + 2 mu) / phsDensity) at its maximum over the
+       elements' materials: the wave dt type's speed of sound
+       (UpdateElement.c:62-67, a running maximum that never changes after
+       the first element pass) */
+    double c_wave;
     uint32_t timeStep;
     int elem_variant;   /* SNH_ELEM_* */
     int node_band;      /* StepArgs::node_band */
@@ -2096,7 +2104,14 @@ static StepArgs make_args(snh_context* c) {
     a.gravity = c->cfg.gravity;
     a.demf = c->cfg.demf;
     a.strain_inert = c->cfg.strain_inert;
+    a.c_wave = c->c_wave;
     return a;
+}
+
+/* Snac_UpdateElementMomentum's running maximum of Vp (UpdateElement.c:62-67) */
+static double wave_speed_max(double cur, const snh_material& m) {
+    const double Vp = sqrt((m.lambda + 2.0 * m.mu) / m.phsDensity);
+    return Vp > cur ? Vp : cur;
 }
 
 static void free_all(snh_context* c) {
@@ -2126,8 +2141,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         if (cfg->precision != SNH_PRECISION_FP64)
             return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 runs on the FP64 path");
     }
-    if (cfg->dtType == SNH_DT_WAVE) return set_err(SNH_EUNSUPPORTED, "dtType wave is not implemented on the device");
-    if (cfg->dtType != SNH_DT_DYNAMIC && cfg->dtType != SNH_DT_CONSTANT) return set_err(SNH_EINVAL, "bad dtType");
+    if (cfg->dtType != SNH_DT_DYNAMIC && cfg->dtType != SNH_DT_CONSTANT && cfg->dtType != SNH_DT_WAVE)
+        return set_err(SNH_EINVAL, "bad dtType");
     if (cfg->precision != SNH_PRECISION_FP64 && cfg->precision != SNH_PRECISION_MIXED)
         return set_err(SNH_EINVAL, "bad precision");
     if (cfg->precision == SNH_PRECISION_MIXED) {
@@ -2156,6 +2171,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     if (!c) return set_err(SNH_ENOMEM, "out of host memory");
     c->cfg = *cfg;
     c->mats.assign(cfg->materials, cfg->materials + cfg->nmaterials);
+    c->c_wave = wave_speed_max(0.0, c->mats[0]);   /* every element has material 0 until told otherwise */
     if (cfg->nbcs) c->bcs.assign(cfg->bcs, cfg->bcs + cfg->nbcs);
     c->cfg.materials = c->mats.data();
     c->cfg.bcs = c->bcs.empty() ? nullptr : c->bcs.data();
@@ -2372,6 +2388,11 @@ extern "C" int snh_set_element_material(snh_context* c, const uint8_t* mat, size
     if (!c->mat) HIPCHK(hipMalloc((void**)&c->mat, nelem));
     HIPCHK(hipMemcpyAsync(c->mat, mat, nelem, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    bool used[SNH_MAX_MATERIALS] = {};
+    for (size_t e = 0; e < nelem; e++) used[mat[e]] = true;
+    c->c_wave = 0.0;
+    for (size_t q = 0; q < c->mats.size(); q++)
+        if (used[q]) c->c_wave = wave_speed_max(c->c_wave, c->mats[q]);
     return SNH_OK;
 }
 

@@ -137,6 +137,7 @@ struct StepArgs {
     int32_t dtType;
     int32_t node_band;      /* k-planes per band of the node passes' order, 0 = plain */
     double gravity, demf, strain_inert;
+    double c_wave;          /* the wave dt type's speed of sound: max Vp over the elements */
 };
 
 /* kinematics of one tetrahedron from its 4 node coordinates and velocities,

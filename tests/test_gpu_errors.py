@@ -26,7 +26,15 @@ def test_unsupported_options_are_refused():
     _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "two nodes wide")
     p = P.elastic_box_compression(elements=(4, 3, 4))
     p.dtType = _abi.DT_WAVE
-    _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "wave")
+    p.precision = _abi.PRECISION_MIXED
+    _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "dynamic dt type only")
+    p = P.elastic_box_compression(elements=(4, 3, 4))
+    p.dtType = _abi.DT_WAVE
+    c = Context(p).Construct().Build().Initialise()
+    try:
+        _raises(lambda: c.adjoint_begin(_abi.PARAM_LOG_VISCOSITY, 10), _abi.SNH_EUNSUPPORTED, "dtType wave")
+    finally:
+        c.Destroy()
     p = P.elastic_box_compression(elements=(4, 3, 4))
     p.materials[0].beta = 1e-11
     _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "beta")

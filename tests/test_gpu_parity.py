@@ -191,3 +191,33 @@ def test_power_law_arrhenius_maxwell(variant):
     _check(f"power_law[{variant}]", ref, got, rsc, gsc, fields=fields)
     v = ref["viscosity"]
     assert np.mean((v > m.vis_min) & (v < m.vis_max)) > 0.5
+
+
+@pytest.mark.parametrize("variant", [0, 3], ids=["tet_lane", "tet_wave"])
+def test_wave_dt_type(variant):
+    """dtType wave (Context.c:1255-1259): dt = 0.5 minLength / Vp, the mass
+    rho V / 4 per tet (Force.c:138-144)."""
+    from snac_amd import _abi
+    p = P.elastic_box_compression(elements=(8, 4, 8))
+    p.dtType = _abi.DT_WAVE
+    ref, got, rsc, gsc = run_both(p, 40, variant=variant)
+    _check(f"wave[{variant}]", ref, got, rsc, gsc)
+    assert rsc["speedOfSound"] == gsc["speedOfSound"] > 0.0
+
+
+def test_wave_dt_type_takes_the_fastest_material():
+    """Two materials: the speed of sound is the largest Vp over the elements'
+    materials (UpdateElement.c:62-67), not material 0's."""
+    from snac_amd import _abi
+    p = P.mohr_coulomb_shear(elements=(8, 6, 6))
+    p.dtType = _abi.DT_WAVE
+    fast = P.MaterialSpec(rheology=_abi.RHEOLOGY_ELASTIC, alpha=0.0, beta=0.0, lam=6.0e10, mu=6.0e10,
+                          density=2500.0)
+    p.materials = [p.materials[0], fast]
+    ni, nj, nk = p.elements
+    mat = np.zeros((nk, nj, ni), dtype=np.uint8)
+    mat[:, nj // 2:, :] = 1
+    p.element_material = mat.reshape(-1)
+    ref, got, rsc, gsc = run_both(p, 30)
+    _check("wave_two_materials", ref, got, rsc, gsc)
+    assert gsc["speedOfSound"] == rsc["speedOfSound"] == pytest.approx((1.8e11 / 2500.0) ** 0.5, rel=1e-6)
