@@ -843,7 +843,7 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
     if (active) {
 #pragma unroll
         for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + ec] = st[c];
-        if (a.dens) store_density(a, m, nd, t, ec);
+        if (a.tpre) store_density(a, m, nd, t, ec);
     }
 }
 
@@ -1129,7 +1129,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
             a.diag[3 * E + e] = totalVolume > 0.0 ? depls / totalVolume : 0.0;
         }
     }
-    if (a.dens)
+    if (a.tpre)
         for (int t = 0; t < NTET; t++) store_density_at(a, m, ei, ej, ek, t, e);
 }
 
@@ -1183,6 +1183,36 @@ __global__ void k_volumes(StepArgs a, double* __restrict__ out) {
  * k_node: Snac_EP_LoopNodesMomentum
  * ------------------------------------------------------------------------ */
 #define GETSIGN(val) ((val) < 0 ? -1.0 : (val) > 0 ? +1.0 : 0.0)
+
+/* the tet densities of Snac_Stress with the pressure-dependent term
+   (Stress.c:147-170), for contexts with a material whose beta != 0: the
+   element pressure is the mean over its tets of the mean of each final tet
+   stress, and density = phsDensity (1 - alpha (avgTemp - reftemp) + beta
+   pressure), avgTemp the tet's mean nodal temperature before the BCs with
+   the temperature plugin, 0 without.  Runs after the element pass. */
+template <bool MM, bool TH>
+__global__ void __launch_bounds__(GEO_BLOCK) k_density(StepArgs a) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const size_t E = (size_t)a.E;
+    double pressure = 0.0;
+#pragma unroll
+    for (int t = 0; t < NTET; t++)
+        pressure += snh_div3(a.stress_out[((size_t)0 * NTET + t) * E + e] + a.stress_out[((size_t)1 * NTET + t) * E + e] +
+                             a.stress_out[((size_t)2 * NTET + t) * E + e]);
+    pressure /= (double)NTET;
+    const DevMaterial& m = mat_of<MM>(a, e);
+    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+#pragma unroll
+    for (int t = 0; t < NTET; t++) {
+        double avg = 0.0;
+        if constexpr (TH) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) avg += a.tpre[hex_node_index(a, ei, ej, ek, kTetraToNode[t][q])] / 4.0;
+        }
+        a.dens[(size_t)t * E + e] = m.phsDensity * (1.0 - m.alpha * (avg - m.reftemp) + m.beta * pressure);
+    }
+}
 
 /* node handled by gather thread g.  Rows of I nodes are taken in bands of
    node_band k-planes, j outer and k inner within a band: the four node rows
@@ -1938,6 +1968,9 @@ struct snh_context {
        (UpdateElement.c:62-67, a running maximum that never changes after
        the first element pass) */
     double c_wave;
+    /* a material with beta != 0: tet densities depend on the pressure
+       (k_density after every element pass) */
+    bool beta_dens;
     uint32_t timeStep;
     int elem_variant;   /* SNH_ELEM_* */
     int node_band;      /* StepArgs::node_band */
@@ -2043,6 +2076,7 @@ static void fill_dev_material(const snh_material& s, DevMaterial& d) {
     d.linear_visc = (s.srexponent == 1.0 && s.activationE == 0.0);
     d.phsDensity = s.phsDensity;
     d.alpha = s.alpha;
+    d.beta = s.beta;
     d.kcond = s.thermal_conduct;
     d.cp = s.heatCapacity;
 }
@@ -2093,7 +2127,7 @@ static StepArgs make_args(snh_context* c) {
     a.tpre = c->thermal ? c->tpre : nullptr;
     a.hflux = c->thermal ? c->hflux : nullptr;
     a.tstress = c->thermal && c->thermal_stress ? c->tstress : nullptr;
-    a.dens = c->thermal ? c->dens : nullptr;
+    a.dens = (c->thermal || c->beta_dens) ? c->dens : nullptr;
     a.tbc = c->thermal ? c->tbc : nullptr;
     a.tbcv = c->thermal ? c->tbcv : nullptr;
     a.I = c->I; a.J = c->J; a.K = c->K;
@@ -2158,8 +2192,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     if (vol > 2.0e9) return set_err(SNH_EINVAL, "mesh too large for 32-bit node indices");
     for (unsigned q = 0; q < cfg->nmaterials; q++) {
         const snh_material& m = cfg->materials[q];
-        if (m.beta != 0.0)
-            return set_err(SNH_EUNSUPPORTED, "material %u: beta != 0 (pressure-dependent density) not on the device",
+        if (m.beta != 0.0 && cfg->precision != SNH_PRECISION_FP64)
+            return set_err(SNH_EUNSUPPORTED, "material %u: beta != 0 (pressure-dependent density) runs on the FP64 path",
                            q);
         if (m.nsegments > SNH_MAX_SEGMENTS) return set_err(SNH_EINVAL, "material %u: too many segments", q);
         for (unsigned s = 0; s <= m.nsegments; s++)
@@ -2172,6 +2206,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     c->cfg = *cfg;
     c->mats.assign(cfg->materials, cfg->materials + cfg->nmaterials);
     c->c_wave = wave_speed_max(0.0, c->mats[0]);   /* every element has material 0 until told otherwise */
+    c->beta_dens = false;
+    for (const snh_material& m : c->mats) c->beta_dens = c->beta_dens || m.beta != 0.0;
     if (cfg->nbcs) c->bcs.assign(cfg->bcs, cfg->bcs + cfg->nbcs);
     c->cfg.materials = c->mats.data();
     c->cfg.bcs = c->bcs.empty() ? nullptr : c->bcs.data();
@@ -2223,6 +2259,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     ALLOC(c->scal, sizeof(DevScalars));
     ALLOC(c->dmat, SNH_MAX_MATERIALS * sizeof(DevMaterial));
     ALLOC(c->dbc, 256 * sizeof(DevBC));
+    if (c->beta_dens) ALLOC(c->dens, 10 * E * sizeof(double));
     if (cfg->topo_kappa > 0) {
         const size_t P = (size_t)c->I * c->K;
         c->topo_parts = (int)((P + TOPO_PREP_BLOCK - 1) / TOPO_PREP_BLOCK);
@@ -2506,8 +2543,22 @@ static int launch_fin(snh_context* c) {
 
 static int launch_elem_args(snh_context* c, StepArgs a);
 static int launch_elem(snh_context* c) {
-    KTimer kt(c, KK_ELEM);
-    return launch_elem_args(c, make_args(c));
+    int rc;
+    {
+        KTimer kt(c, KK_ELEM);
+        rc = launch_elem_args(c, make_args(c));
+    }
+    if (rc == SNH_OK && c->beta_dens) {
+        /* the pressure-dependent tet densities of this Stress call */
+        StepArgs a = make_args(c);
+        const dim3 g(c->n_geo_blocks), b(GEO_BLOCK);
+        if (a.mat && a.tpre) hipLaunchKernelGGL((k_density<true, true>), g, b, 0, c->stream, a);
+        else if (a.mat) hipLaunchKernelGGL((k_density<true, false>), g, b, 0, c->stream, a);
+        else if (a.tpre) hipLaunchKernelGGL((k_density<false, true>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_density<false, false>), g, b, 0, c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
+    return rc;
 }
 static int launch_elem_args(snh_context* c, StepArgs a) {
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
@@ -2680,7 +2731,7 @@ extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
         HIPCHK(hipMalloc((void**)&c->temp0, N * sizeof(double)));
         HIPCHK(hipMalloc((void**)&c->tpre, N * sizeof(double)));
         HIPCHK(hipMalloc((void**)&c->hflux, 30 * E * sizeof(double)));
-        HIPCHK(hipMalloc((void**)&c->dens, 10 * E * sizeof(double)));
+        if (!c->dens) HIPCHK(hipMalloc((void**)&c->dens, 10 * E * sizeof(double)));
         HIPCHK(hipMalloc((void**)&c->tbc, N));
         HIPCHK(hipMalloc((void**)&c->tbcv, 256 * sizeof(double)));
         HIPCHK(hipMemset(c->hflux, 0, 30 * E * sizeof(double)));

@@ -87,6 +87,8 @@ extern "C" int snh_adjoint_begin(snh_context* c, const snh_adjoint_config* cfg) 
     if (c->adj) return set_err(SNH_ESTATE, "adjoint already begun (snh_adjoint_end first)");
     if (c->cfg.topo_kappa > 0)
         return set_err(SNH_EUNSUPPORTED, "the adjoint does not differentiate topography diffusion (topo_kappa > 0)");
+    if (c->beta_dens)
+        return set_err(SNH_EUNSUPPORTED, "the adjoint takes the tet density as constant (a material has beta != 0)");
     if (c->cfg.dtType == SNH_DT_WAVE)
         return set_err(SNH_EUNSUPPORTED, "the adjoint's mass reverse is the dynamic dt type's (dtType wave)");
     if (cfg->checkpoint_interval < 1) return set_err(SNH_EINVAL, "checkpoint_interval must be >= 1");

@@ -62,8 +62,8 @@ struct DevMaterial {
     /* viscosity */
     double vis_min, vis_max, refvisc, refsrate, reftemp, activationE, srexponent;
     int linear_visc;         /* srexponent == 1 and activationE == 0 */
-    /* temperature plugin */
-    double phsDensity, alpha, kcond, cp;
+    /* temperature plugin; beta: pressure-dependent density (Stress.c:169-170) */
+    double phsDensity, alpha, kcond, cp, beta;
 };
 
 struct DevScalars {

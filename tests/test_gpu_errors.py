@@ -37,6 +37,7 @@ def test_unsupported_options_are_refused():
         c.Destroy()
     p = P.elastic_box_compression(elements=(4, 3, 4))
     p.materials[0].beta = 1e-11
+    p.precision = _abi.PRECISION_MIXED
     _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "beta")
 
 

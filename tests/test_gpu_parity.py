@@ -221,3 +221,22 @@ def test_wave_dt_type_takes_the_fastest_material():
     ref, got, rsc, gsc = run_both(p, 30)
     _check("wave_two_materials", ref, got, rsc, gsc)
     assert gsc["speedOfSound"] == rsc["speedOfSound"] == pytest.approx((1.8e11 / 2500.0) ** 0.5, rel=1e-6)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 3], ids=["tet_lane", "elem_lane", "tet_wave"])
+def test_pressure_dependent_density(variant):
+    """beta != 0 (Stress.c:147-170): the tet density follows the element
+    pressure every step, and the hydrostatic start uses beta too
+    (hydroStaticIC/VariableConditions.c:43-160)."""
+    p = P.elastic_box_compression(elements=(8, 4, 8))
+    p.materials[0].beta = 1.0e-11
+    ref, got, rsc, gsc = run_both(p, 40, variant=variant)
+    _check(f"beta[{variant}]", ref, got, rsc, gsc)
+
+
+def test_pressure_dependent_density_with_temperature():
+    p = P.maxwell_thermal(elements=(6, 4, 6))
+    p.materials[0].beta = 1.0e-11
+    fields = FIELDS + ("temperature",)
+    ref, got, rsc, gsc = run_both(p, 30, fields=fields)
+    _check("beta_thermal", ref, got, rsc, gsc, fields=fields)
