@@ -1383,7 +1383,7 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
  * order) whose gradient reaches the criterion per block.  k_topo (one
  * block) finishes the reductions and sweeps the diagonals with the heights
  * of diagonal d-1 in LDS, prefetching records several diagonals ahead
- * (they come from other XCDs' writes, about 2 us away); a
+ * (written by k_topo_prep on all XCDs just before); a
  * division whose divisor the sweep does not change is done as
  * snh_div_by with the stored reciprocal (correctly rounded, as the
  * reference's).  Both run after k_node with its arguments: xc the old
