@@ -123,7 +123,9 @@ typedef struct snh_config {
     double   strain_inert;        /* dynamic mass-scaling factor                */
     double   topo_kappa;          /* top-surface diffusivity (0 = off; the
                                      reference's default is 5e-6), Cartesian
-                                     meshes more than two nodes wide in x and z */
+                                     meshes; two nodes wide in x (z) it
+                                     diffuses along z (x) only, as
+                                     UpdateNode.c:494-499 does             */
     double   topoGradCriterion;   /* diffusion runs once the running maximum of
                                      |grad y| over the top surface reaches it
                                      (the reference's default 1e-16)        */

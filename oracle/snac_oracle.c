@@ -305,10 +305,6 @@ orc_context* orc_create(const snh_config* cfg) {
         fail("orc_create: bad material list");
         return NULL;
     }
-    if (cfg->topo_kappa > 0 && (cfg->meshSizeI < 3 || cfg->meshSizeK < 3)) {
-        fail("orc_create: topo_kappa > 0 on a mesh two nodes wide (diffTopo_alongX/Z) is not restated");
-        return NULL;
-    }
     orc_context* c = (orc_context*)calloc(1, sizeof *c);
     if (!c) return NULL;
     c->cfg = *cfg;
@@ -1284,8 +1280,13 @@ int orc_update_node_preprocess(orc_context* c, uint32_t n, double inertialMass, 
 }
 
 /* ---------------------------------------------------------------------------
- * Topography diffusion of the top surface (topo_kappa > 0), Cartesian meshes
- * more than two nodes wide in x and z.  The reference runs it per node inside
+ * Topography diffusion of the top surface (topo_kappa > 0), Cartesian meshes.
+ * More than two nodes wide in x and z the surface is diffused in two
+ * dimensions; two nodes wide in x (resp. z) and more in the other direction
+ * it is diffused along z (resp. x) only, as diffTopo_alongZ/_alongX,
+ * getMaxTopoGrad_alongZ/_alongX and AdjustEdgeTopo's last two branches do
+ * (UpdateNode.c:248-251,262-357,494-499,508-645,686-708,811-856); two nodes
+ * wide in both it is left alone.  The reference runs it per node inside
  * the second node pass, in node order, after the node's coordinate update:
  * neighbours with a smaller index already hold their new coordinates (and
  * diffused heights), those with a larger one still hold the old ones.  The
@@ -1293,63 +1294,108 @@ int orc_update_node_preprocess(orc_context* c, uint32_t n, double inertialMass, 
  * Geometry/src/IJK6Topology.c:283-343, fixed positions): 0 +x, 2 +z, 3 -x,
  * 5 -z.
  * ------------------------------------------------------------------------- */
-static int top_interior(const orc_context* c, unsigned i, unsigned k) {
-    return i > 0 && i < c->I - 1 && k > 0 && k < c->K - 1;
+enum { TOPO_SURFACE = 0, TOPO_ALONG_Z = 1, TOPO_ALONG_X = 2, TOPO_NONE = 3 };
+
+/* the branch the reference takes on nodeGlobal3DCounts (UpdateNode.c:
+   196,248-251) */
+static int topo_mode(const orc_context* c) {
+    if (c->I > 2 && c->K > 2) return TOPO_SURFACE;
+    if (c->I == 2 && c->K > 2) return TOPO_ALONG_Z;
+    if (c->K == 2 && c->I > 2) return TOPO_ALONG_X;
+    return TOPO_NONE;
 }
 
-/* getMaxTopoGrad, UpdateNode.c:161-260 (Cartesian branch) */
+/* a top node whose height the finite-difference cell defines */
+static int top_interior(const orc_context* c, unsigned i, unsigned k) {
+    const int inx = i > 0 && i < c->I - 1, inz = k > 0 && k < c->K - 1;
+    switch (topo_mode(c)) {
+    case TOPO_SURFACE: return inx && inz;
+    case TOPO_ALONG_Z: return inz;
+    case TOPO_ALONG_X: return inx;
+    default: return 0;
+    }
+}
+
+/* getMaxTopoGrad and its _alongZ/_alongX variants, UpdateNode.c:161-357
+   (Cartesian branches) */
 static void topo_max_grad(orc_context* c, unsigned n) {
     const unsigned i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
     if (c->timeStep <= 1) return;
     if (j < c->J - 1) return;
     if (!top_interior(c, i, k)) return;
+    const int mode = topo_mode(c);
     const double* X0 = c->coord[n];
-    const double* xR = c->coord[n + 1];
-    const double* xF = c->coord[n + c->I * c->J];
-    const double dydx = (xR[1] - X0[1]) / (xR[0] - X0[0]);
-    const double dydz = (xF[1] - X0[1]) / (xF[2] - X0[2]);
-    const double g = sqrt(dydx * dydx + dydz * dydz);
+    double g;
+    if (mode == TOPO_SURFACE) {
+        const double* xR = c->coord[n + 1];
+        const double* xF = c->coord[n + c->I * c->J];
+        const double dydx = (xR[1] - X0[1]) / (xR[0] - X0[0]);
+        const double dydz = (xF[1] - X0[1]) / (xF[2] - X0[2]);
+        g = sqrt(dydx * dydx + dydz * dydz);
+    } else if (mode == TOPO_ALONG_Z) {
+        const double* xF = c->coord[n + c->I * c->J];
+        const double dydz = (xF[1] - X0[1]) / (xF[2] - X0[2]);
+        g = sqrt(dydz * dydz);
+    } else {
+        const double* xR = c->coord[n + 1];
+        const double dydx = (xR[1] - X0[1]) / (xR[0] - X0[0]);
+        g = sqrt(dydx * dydx);
+    }
     c->topoGradMax = c->topoGradMax > g ? c->topoGradMax : g;
 }
 
-/* Snac_UpdateNodeMomentum_DiffTopo and _UpdateInteriorTopo,
-   UpdateNode.c:359-505,647-714 (Cartesian branch): dh = kappa dt lapl(y)
-   from the five-point stencil, added to the node's height */
+/* Snac_UpdateNodeMomentum_DiffTopo (with diffTopo_alongZ/_alongX) and
+   _UpdateInteriorTopo, UpdateNode.c:359-714 (Cartesian branches):
+   dh = kappa dt lapl(y) from the five-point stencil (three-point along one
+   direction), added to the node's height */
 static void topo_diffuse(orc_context* c, unsigned n) {
     const unsigned i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
     if (c->timeStep <= 1) return;
     if (j < c->J - 1 || c->topoGradMax < c->cfg.topoGradCriterion) return;
     if (!top_interior(c, i, k)) return;
+    const int mode = topo_mode(c);
     const unsigned IJ = c->I * c->J;
     double* X0 = c->coord[n];
-    const double* xR = c->coord[n + 1];
-    const double* xF = c->coord[n + IJ];
-    const double* xL = c->coord[n - 1];
-    const double* xB = c->coord[n - IJ];
-    const double dydx1 = (xR[1] - X0[1]) / (xR[0] - X0[0]);
-    const double dydx2 = (X0[1] - xL[1]) / (X0[0] - xL[0]);
-    const double d2ydx2 = (dydx1 - dydx2) / (0.5 * (xR[0] - xL[0]));
-    const double dydz1 = (xF[1] - X0[1]) / (xF[2] - X0[2]);
-    const double dydz2 = (X0[1] - xB[1]) / (X0[2] - xB[2]);
-    const double d2ydz2 = (dydz1 - dydz2) / (0.5 * (xF[2] - xB[2]));
-    const double lapl_topo = d2ydx2 + d2ydz2;
+    double d2ydx2 = 0.0, d2ydz2 = 0.0;
+    if (mode != TOPO_ALONG_Z) {
+        const double* xR = c->coord[n + 1];
+        const double* xL = c->coord[n - 1];
+        const double dydx1 = (xR[1] - X0[1]) / (xR[0] - X0[0]);
+        const double dydx2 = (X0[1] - xL[1]) / (X0[0] - xL[0]);
+        d2ydx2 = (dydx1 - dydx2) / (0.5 * (xR[0] - xL[0]));
+    }
+    if (mode != TOPO_ALONG_X) {
+        const double* xF = c->coord[n + IJ];
+        const double* xB = c->coord[n - IJ];
+        const double dydz1 = (xF[1] - X0[1]) / (xF[2] - X0[2]);
+        const double dydz2 = (X0[1] - xB[1]) / (X0[2] - xB[2]);
+        d2ydz2 = (dydz1 - dydz2) / (0.5 * (xF[2] - xB[2]));
+    }
+    const double lapl_topo = mode == TOPO_SURFACE ? d2ydx2 + d2ydz2
+                           : (mode == TOPO_ALONG_Z ? d2ydz2 : d2ydx2);
     const double dh = c->cfg.topo_kappa * c->dt * lapl_topo;
     X0[1] += dh;
 }
 
 /* Snac_UpdateNodeMomentum_AdjustEdgeTopo, UpdateNode.c:716-857 (Cartesian
-   branch): an edge node takes the height of its inner neighbour, a corner the
-   mean of its two */
+   branches): an edge node takes the height of its inner neighbour, a corner
+   of the two-dimensional surface the mean of its two */
 static void topo_adjust_edge(orc_context* c, unsigned n) {
     const unsigned i = n % c->I, j = (n / c->I) % c->J, k = n / (c->I * c->J);
     if (c->timeStep <= 1) return;
     if (j < c->J - 1 || c->topoGradMax < c->cfg.topoGradCriterion) return;
+    const int mode = topo_mode(c);
+    if (mode == TOPO_NONE) return;
     const unsigned IJ = c->I * c->J;
     const double *xN1 = NULL, *xN2 = NULL;
-    if (i == 0) xN1 = c->coord[n + 1];
-    else if (i == c->I - 1) xN1 = c->coord[n - 1];
-    if (k == 0) xN2 = c->coord[n + IJ];
-    else if (k == c->K - 1) xN2 = c->coord[n - IJ];
+    if (mode != TOPO_ALONG_Z) {
+        if (i == 0) xN1 = c->coord[n + 1];
+        else if (i == c->I - 1) xN1 = c->coord[n - 1];
+    }
+    if (mode != TOPO_ALONG_X) {
+        if (k == 0) xN2 = c->coord[n + IJ];
+        else if (k == c->K - 1) xN2 = c->coord[n - IJ];
+    }
     double* X0 = c->coord[n];
     if (xN1 && !xN2) X0[1] = xN1[1];
     else if (!xN1 && xN2) X0[1] = xN2[1];

@@ -1373,7 +1373,13 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
  * -x and -z neighbours and the old coordinates of its +x and +z ones (slots
  * 3, 5 and 0, 2 of IJK6Topology_BuildNeighbours), so the final height of
  * node (i, k) depends on (i-1, k) and (i, k-1) only: the surface is swept
- * by anti-diagonals d = i + k.
+ * by anti-diagonals d = i + k.  Two nodes wide in x (resp. z) and more in
+ * the other direction, the reference diffuses along z (resp. x) only
+ * (diffTopo_alongZ/_alongX, getMaxTopoGrad_alongZ/_alongX and the last two
+ * branches of AdjustEdgeTopo, UpdateNode.c:248-251,262-357,494-499,508-645,
+ * 686-708,811-856): a node then depends on (i, k-1) (resp. (i-1, k)) alone,
+ * which the same diagonal order respects; two nodes wide in both it does
+ * nothing.
  *
  * k_topo_prep (one thread per top node) computes everything the sweep does
  * not change -- the node's gradient for the running maximum, the
@@ -1392,13 +1398,24 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
 #define TOPO_REC 11          /* doubles per top-node record            */
 #define TOPO_PREP_BLOCK 256
 
+/* which of the reference's branches applies (on nodeGlobal3DCounts) */
+enum { TOPO_SURFACE = 0, TOPO_ALONG_Z = 1, TOPO_ALONG_X = 2, TOPO_NONE = 3 };
+
 struct TopoArgs {
     double* rec;             /* [TOPO_REC][I K] records, diagonal order */
     double* pmax;            /* [nparts] block maxima of |grad y|       */
     int* pfirst;             /* [nparts] block minima of the first node */
     int nparts;
+    int mode;                /* TOPO_SURFACE, _ALONG_Z or _ALONG_X      */
     double kappa, crit;
 };
+
+static int topo_mode(int I, int K) {
+    if (I > 2 && K > 2) return TOPO_SURFACE;
+    if (I == 2 && K > 2) return TOPO_ALONG_Z;
+    if (K == 2 && I > 2) return TOPO_ALONG_X;
+    return TOPO_NONE;
+}
 
 __device__ __forceinline__ int topo_ilo(int d, int K) { return d - (K - 1) > 0 ? d - (K - 1) : 0; }
 /* first record of diagonal d: the top nodes with i + k < d on an I x K
@@ -1427,29 +1444,39 @@ __global__ void __launch_bounds__(TOPO_PREP_BLOCK) k_topo_prep(StepArgs a, TopoA
 #pragma unroll
         for (int q = 0; q < TOPO_REC; q++) r[q] = 0.0;
         r[0] = a.xn[N + n];
-        if (i > 0 && i < I - 1 && k > 0 && k < K - 1) {
-            const double x0 = a.xn[n], y0 = r[0], z0 = a.xn[2 * N + n];
-            const double xR = a.xc[n + 1], yR = a.xc[N + n + 1];
-            const double yF = a.xc[N + n + IJ], zF = a.xc[2 * N + n + IJ];
-            const double xL = a.xn[n - 1], zB = a.xn[2 * N + n - IJ];
-            const double dydx1 = (yR - y0) / (xR - x0);
-            const double dydz1 = (yF - y0) / (zF - z0);
-            r[1] = dydx1;
-            r[2] = x0 - xL;
-            r[3] = 1.0 / r[2];
-            r[4] = 0.5 * (xR - xL);
-            r[5] = 1.0 / r[4];
-            r[6] = dydz1;
-            r[7] = z0 - zB;
-            r[8] = 1.0 / r[7];
-            r[9] = 0.5 * (zF - zB);
-            r[10] = 1.0 / r[9];
+        const bool inx = i > 0 && i < I - 1, inz = k > 0 && k < K - 1;
+        const bool alx = t.mode != TOPO_ALONG_Z, alz = t.mode != TOPO_ALONG_X;
+        const bool interior = (!alx || inx) && (!alz || inz);
+        if (interior) {
+            const double y0 = r[0];
+            double dydx1 = 0.0, dydz1 = 0.0;
+            if (alx) {
+                const double x0 = a.xn[n], xL = a.xn[n - 1];
+                const double xR = a.xc[n + 1], yR = a.xc[N + n + 1];
+                dydx1 = (yR - y0) / (xR - x0);
+                r[1] = dydx1;
+                r[2] = x0 - xL;
+                r[3] = 1.0 / r[2];
+                r[4] = 0.5 * (xR - xL);
+                r[5] = 1.0 / r[4];
+            }
+            if (alz) {
+                const double z0 = a.xn[2 * N + n], zB = a.xn[2 * N + n - IJ];
+                const double yF = a.xc[N + n + IJ], zF = a.xc[2 * N + n + IJ];
+                dydz1 = (yF - y0) / (zF - z0);
+                r[6] = dydz1;
+                r[7] = z0 - zB;
+                r[8] = 1.0 / r[7];
+                r[9] = 0.5 * (zF - zB);
+                r[10] = 1.0 / r[9];
+            }
             /* getMaxTopoGrad's dydx, dydz are the same quotients */
-            g = sqrt(dydx1 * dydx1 + dydz1 * dydz1);
+            g = (alx && alz) ? sqrt(dydx1 * dydx1 + dydz1 * dydz1)
+                             : (alx ? sqrt(dydx1 * dydx1) : sqrt(dydz1 * dydz1));
             if (!(g < t.crit)) first = p;
         } else {
-            if (i == 0) r[1] = a.xc[N + n + 1];    /* old height of (1, k)  */
-            if (k == 0) r[6] = a.xc[N + n + IJ];   /* old height of (i, 1)  */
+            if (alx && i == 0) r[1] = a.xc[N + n + 1];    /* old height of (1, k)  */
+            if (alz && k == 0) r[6] = a.xc[N + n + IJ];   /* old height of (i, 1)  */
         }
 #pragma unroll
         for (int q = 0; q < TOPO_REC; q++) t.rec[q * PP + pos] = r[q];
@@ -1542,11 +1569,13 @@ __global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
             const double d2ydx2 = snh_div_by(rc[1] - dydx2, rc[4], rc[5]);
             const double dydz2 = snh_div_by(y0 - yb, rc[7], rc[8]);
             const double d2ydz2 = snh_div_by(rc[6] - dydz2, rc[9], rc[10]);
-            const double lapl_topo = d2ydx2 + d2ydz2;
+            const double lapl_topo = t.mode == TOPO_SURFACE ? d2ydx2 + d2ydz2
+                                   : (t.mode == TOPO_ALONG_Z ? d2ydz2 : d2ydx2);
             const double yi = y0 + kd * lapl_topo;
             /* AdjustEdgeTopo: xN1 along x, xN2 along z, the old height of a
                neighbour not yet updated (+x, +z), the final one otherwise */
-            const bool h1 = (i == 0 || i == I - 1), h2 = (k == 0 || k == K - 1);
+            const bool h1 = t.mode != TOPO_ALONG_Z && (i == 0 || i == I - 1);
+            const bool h2 = t.mode != TOPO_ALONG_X && (k == 0 || k == K - 1);
             const double y1 = (i == 0) ? rc[1] : yl;
             const double y2 = (k == 0) ? rc[6] : yb;
             const double ye = (h1 && !h2) ? y1 : ((!h1 && h2) ? y2 : 0.5 * (y1 + y2));
@@ -2166,8 +2195,6 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     if (cfg->nmaterials < 1 || cfg->nmaterials > SNH_MAX_MATERIALS || !cfg->materials)
         return set_err(SNH_EINVAL, "need 1..%d materials", SNH_MAX_MATERIALS);
     if (cfg->topo_kappa > 0) {
-        if (cfg->meshSizeI < 3 || cfg->meshSizeK < 3)
-            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 on a mesh two nodes wide (diffTopo_alongX/Z)");
         if (cfg->meshSizeI > TOPO_MAX_I)
             return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than %d nodes along x", TOPO_MAX_I);
         if (cfg->meshSizeI > 1024 && cfg->meshSizeK > 1024)
@@ -2661,9 +2688,10 @@ static int launch_node(snh_context* c, int refStep) {
     else
         hipLaunchKernelGGL((k_node<false, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
-    if (c->cfg.topo_kappa > 0 && refStep > 1) {
+    if (c->cfg.topo_kappa > 0 && refStep > 1 && topo_mode(c->I, c->K) != TOPO_NONE) {
         /* before the buffer flip: xc still holds the old coordinates */
         TopoArgs t;
+        t.mode = topo_mode(c->I, c->K);
         t.rec = c->topo_rec;
         t.pmax = c->topo_pmax;
         t.pfirst = c->topo_pfirst;

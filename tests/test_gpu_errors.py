@@ -21,9 +21,9 @@ def _raises(fn, code, text):
 
 
 def test_unsupported_options_are_refused():
-    p = P.elastic_box_compression(elements=(1, 3, 4))
+    p = P.elastic_box_compression(elements=(4100, 1, 2))
     p.topo_kappa = 5e-6
-    _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "two nodes wide")
+    _raises(lambda: Context(p).Construct(), _abi.SNH_EUNSUPPORTED, "nodes along x")
     p = P.elastic_box_compression(elements=(4, 3, 4))
     p.dtType = _abi.DT_WAVE
     p.precision = _abi.PRECISION_MIXED

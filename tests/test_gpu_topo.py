@@ -1,7 +1,8 @@
 """Topography diffusion on the GPU (k_topo, the anti-diagonal sweep of the
 top surface) against the oracle's node-order restatement: bit-identical
 coordinates, velocities and stresses, and the same running maximum of
-|grad y|."""
+|grad y|, on surfaces more than two nodes wide and on the quasi
+two-dimensional meshes the reference diffuses along one direction."""
 import numpy as np
 import pytest
 
@@ -55,28 +56,62 @@ def test_wide_surfaces(elements):
     assert gsc["topoGradMax"] > 0.0
 
 
-def test_parabola_at_rest_matches_the_oracle():
-    """The oracle's pinned case (tests/test_oracle_topo.py) on the device."""
-    p = P.elastic_box_compression(elements=(8, 3, 5), extent_km=(8.0, 3.0, 5.0))
-    p.bcs, p.gravity, p.hydrostatic, p.topo_kappa = [], 0.0, False, 0.5
+def _surface_run(p, a, b, steps, fields):
+    """Oracle and device from the same raised top surface y += a x^2 + b z^2."""
     ni, nj, nk = p.nodes
     o = Oracle(p, threads=2)
     coord = o.state()["coord"].copy()
     top = np.array([i + ni * (nj - 1) + ni * nj * k for k in range(nk) for i in range(ni)])
-    coord[top, 1] += 1.0e-6 * coord[top, 0] ** 2 + 2.0e-6 * coord[top, 2] ** 2
+    coord[top, 1] += a * coord[top, 0] ** 2 + b * coord[top, 2] ** 2
     o.set_state(coord=coord)
     o.timestep_zero()
-    o.step(6)
+    o.step(steps)
     ref, rsc = o.state(), o.scalars()
     o.close()
     c = Context(p).Construct().Build()
     c.set_state(coord=coord)
     c.timestep_zero()
-    c.Execute(6)
-    got, gsc = c.state(fields=("coord", "velocity")), c.scalars()
+    c.Execute(steps)
+    got, gsc = c.state(fields=fields), c.scalars()
     c.Destroy()
-    _check_equal("parabola", ref, got, rsc, gsc, fields=("coord", "velocity"))
+    return ref, got, rsc, gsc, coord, top
+
+
+def test_parabola_at_rest_matches_the_oracle():
+    """The oracle's pinned case (tests/test_oracle_topo.py) on the device."""
+    p = P.elastic_box_compression(elements=(8, 3, 5), extent_km=(8.0, 3.0, 5.0))
+    p.bcs, p.gravity, p.hydrostatic, p.topo_kappa = [], 0.0, False, 0.5
+    fields = ("coord", "velocity")
+    ref, got, rsc, gsc, coord, top = _surface_run(p, 1.0e-6, 2.0e-6, 6, fields)
+    _check_equal("parabola", ref, got, rsc, gsc, fields=fields)
     assert np.any(got["coord"][top, 1] != coord[top, 1])
+
+
+@pytest.mark.parametrize("elements,bcs", [((1, 3, 12), False), ((12, 3, 1), False),
+                                          ((1, 3, 12), True), ((12, 3, 1), True),
+                                          ((1, 3, 1), False), ((3000, 1, 1), False)],
+                         ids=["along_z", "along_x", "along_z_compressed", "along_x_compressed",
+                              "two_by_two", "along_x_3001"])
+def test_quasi_two_dimensional_meshes(elements, bcs):
+    """Two nodes wide in x (resp. z): the reference's diffTopo_alongZ
+    (resp. _alongX) branch; two nodes wide in both: no diffusion."""
+    ex, ey, ez = elements
+    p = P.elastic_box_compression(elements=elements, extent_km=(float(ex), float(ey), float(ez)))
+    p.gravity, p.hydrostatic = 0.0, False
+    if not bcs:
+        p.bcs = []
+    p.topo_kappa = 3.0e-3      # kappa dt / h^2 ~ 0.09 with dt ~ 2.9e7 s, h = 1 km: stable
+    a, b = 50.0 / (1e3 * ex) ** 2, 50.0 / (1e3 * ez) ** 2     # a 50 m high bowl
+    fields = FIELDS if bcs else ("coord", "velocity")
+    ref, got, rsc, gsc, coord, top = _surface_run(p, a, b, 8, fields)
+    _check_equal(f"quasi2d{elements}{'_bc' if bcs else ''}", ref, got, rsc, gsc, fields=fields)
+    if ex == 1 and ez == 1:
+        assert gsc["topoGradMax"] == 0.0
+        if not bcs:
+            assert np.array_equal(got["coord"], coord)
+    else:
+        assert gsc["topoGradMax"] > 0.0
+        assert np.any(got["coord"][top, 1] != coord[top, 1])
 
 
 def test_granular_entry_points_match_the_step():
@@ -106,7 +141,6 @@ def _refused(make, text):
 
 def test_refused_where_not_built():
     base = lambda el: _with_topo(P.elastic_box_compression(elements=el))
-    _refused(lambda: Context(base((1, 4, 8))).Construct(), "two nodes wide")
     _refused(lambda: Context(base((4100, 1, 2))).Construct(), "nodes along x")
     mixed = base((6, 3, 6))
     mixed.precision = _abi.PRECISION_MIXED
