@@ -1378,8 +1378,8 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
  * (diffTopo_alongZ/_alongX, getMaxTopoGrad_alongZ/_alongX and the last two
  * branches of AdjustEdgeTopo, UpdateNode.c:248-251,262-357,494-499,508-645,
  * 686-708,811-856): a node then depends on (i, k-1) (resp. (i-1, k)) alone,
- * which the same diagonal order respects; two nodes wide in both it does
- * nothing.
+ * so each of the two lines is a chain of its own, which k_topo_line runs
+ * one thread per line; two nodes wide in both it does nothing.
  *
  * k_topo_prep (one thread per top node) computes everything the sweep does
  * not change -- the node's gradient for the running maximum, the
@@ -1397,12 +1397,14 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
  * ------------------------------------------------------------------------ */
 #define TOPO_REC 11          /* doubles per top-node record            */
 #define TOPO_PREP_BLOCK 256
+#define TOPO_LINE_REC 8      /* doubles per record in the one-dimensional modes */
 
 /* which of the reference's branches applies (on nodeGlobal3DCounts) */
 enum { TOPO_SURFACE = 0, TOPO_ALONG_Z = 1, TOPO_ALONG_X = 2, TOPO_NONE = 3 };
 
 struct TopoArgs {
-    double* rec;             /* [TOPO_REC][I K] records, diagonal order */
+    double* rec;             /* [TOPO_REC][I K] records, diagonal order;
+                                one-dimensional modes: [lines][L][TOPO_LINE_REC] */
     double* pmax;            /* [nparts] block maxima of |grad y|       */
     int* pfirst;             /* [nparts] block minima of the first node */
     int nparts;
@@ -1478,8 +1480,19 @@ __global__ void __launch_bounds__(TOPO_PREP_BLOCK) k_topo_prep(StepArgs a, TopoA
             if (alx && i == 0) r[1] = a.xc[N + n + 1];    /* old height of (1, k)  */
             if (alz && k == 0) r[6] = a.xc[N + n + IJ];   /* old height of (i, 1)  */
         }
+        if (t.mode == TOPO_SURFACE) {
 #pragma unroll
-        for (int q = 0; q < TOPO_REC; q++) t.rec[q * PP + pos] = r[q];
+            for (int q = 0; q < TOPO_REC; q++) t.rec[q * PP + pos] = r[q];
+        } else {
+            /* one line per row (along x) or column (along z): the height and
+               the direction's five fields, 64 bytes per node in line order */
+            const size_t lpos = alx ? (size_t)p : (size_t)i * K + k;
+            double* o = t.rec + TOPO_LINE_REC * lpos;
+            const int f0 = alx ? 1 : 6;
+            o[0] = r[0];
+#pragma unroll
+            for (int q = 0; q < 5; q++) o[1 + q] = r[f0 + q];
+        }
     }
     smax[threadIdx.x] = g;
     sfirst[threadIdx.x] = first;
@@ -1499,22 +1512,14 @@ __global__ void __launch_bounds__(TOPO_PREP_BLOCK) k_topo_prep(StepArgs a, TopoA
     }
 }
 
-/* BLOCK: the most threads the sweep may be launched with (a diagonal's
-   nodes, at most min(I, K)); PF: diagonals prefetched ahead -- deeper with
-   the smaller block, whose waves may hold more registers */
-template <int BLOCK, int PF>
-__global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
-    constexpr int TOPO_PF = PF;
-    __shared__ double yd[2][TOPO_MAX_I];
-    __shared__ double rmax[BLOCK];
-    __shared__ int rfirst[BLOCK];
-    const int I = a.I, J = a.J, K = a.K, IJ = I * J, P = I * K;
-    const int tid = threadIdx.x, B = blockDim.x;
-    const size_t N = (size_t)a.N, PP = (size_t)P;
+/* The running maximum over k_topo_prep's block partials (one block):
+   stores the new maximum and returns the first node, in node order, from
+   which the diffusion runs -- 0 if an earlier step already reached the
+   criterion, I K if this one does not.  rmax, rfirst: blockDim.x entries,
+   a power of two. */
+__device__ int topo_reduce(const StepArgs& a, const TopoArgs& t, double* rmax, int* rfirst) {
+    const int tid = threadIdx.x, B = blockDim.x, P = a.I * a.K;
     const double prev = a.scal->topoGradMax;
-    const double kd = t.kappa * a.scal->dt;
-
-    /* the running maximum: reaches the criterion at node pfirst (node order) */
     double m = 0.0;
     int f = P;
     for (int q = tid; q < t.nparts; q += B) {
@@ -1533,6 +1538,25 @@ __global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
     }
     const int pfirst = (prev < t.crit) ? rfirst[0] : 0;
     if (tid == 0) a.scal->topoGradMax = prev > rmax[0] ? prev : rmax[0];
+    return pfirst;
+}
+
+/* BLOCK: the most threads the sweep may be launched with (a diagonal's
+   nodes, at most min(I, K)); PF: diagonals prefetched ahead -- deeper with
+   the smaller block, whose waves may hold more registers */
+template <int BLOCK, int PF>
+__global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
+    constexpr int TOPO_PF = PF;
+    __shared__ double yd[2][TOPO_MAX_I];
+    __shared__ double rmax[BLOCK];
+    __shared__ int rfirst[BLOCK];
+    const int I = a.I, J = a.J, K = a.K, IJ = I * J, P = I * K;
+    const int tid = threadIdx.x;
+    const size_t N = (size_t)a.N, PP = (size_t)P;
+    const double kd = t.kappa * a.scal->dt;
+
+    /* the running maximum: reaches the criterion at node pfirst (node order) */
+    const int pfirst = topo_reduce(a, t, rmax, rfirst);
 
     /* the sweep; thread tid holds node i = ilo(d) + tid of diagonal d.  The
        records of diagonal d + TOPO_PF are loaded while diagonal d is
@@ -1569,13 +1593,11 @@ __global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
             const double d2ydx2 = snh_div_by(rc[1] - dydx2, rc[4], rc[5]);
             const double dydz2 = snh_div_by(y0 - yb, rc[7], rc[8]);
             const double d2ydz2 = snh_div_by(rc[6] - dydz2, rc[9], rc[10]);
-            const double lapl_topo = t.mode == TOPO_SURFACE ? d2ydx2 + d2ydz2
-                                   : (t.mode == TOPO_ALONG_Z ? d2ydz2 : d2ydx2);
+            const double lapl_topo = d2ydx2 + d2ydz2;
             const double yi = y0 + kd * lapl_topo;
             /* AdjustEdgeTopo: xN1 along x, xN2 along z, the old height of a
                neighbour not yet updated (+x, +z), the final one otherwise */
-            const bool h1 = t.mode != TOPO_ALONG_Z && (i == 0 || i == I - 1);
-            const bool h2 = t.mode != TOPO_ALONG_X && (k == 0 || k == K - 1);
+            const bool h1 = (i == 0 || i == I - 1), h2 = (k == 0 || k == K - 1);
             const double y1 = (i == 0) ? rc[1] : yl;
             const double y2 = (k == 0) ? rc[6] : yb;
             const double ye = (h1 && !h2) ? y1 : ((!h1 && h2) ? y2 : 0.5 * (y1 + y2));
@@ -1591,6 +1613,66 @@ __global__ void __launch_bounds__(BLOCK) k_topo(StepArgs a, TopoArgs t) {
                 for (int q = 0; q < TOPO_REC; q++) ring[r][q] = t.rec[q * PP + pos];
             }
             __syncthreads();
+        }
+    }
+}
+
+/* The one-dimensional modes: two lines of L nodes (rows k = 0, 1 along x
+   when the mesh is two nodes wide in z, columns i = 0, 1 along z when it is
+   two wide in x), each a chain in which node s depends on node s-1 of its
+   own line only.  One thread per line keeps the previous final height in a
+   register -- no LDS hand-over, no barrier -- and prefetches the records of
+   PF nodes ahead; the node update is the sweep's, restricted to the line's
+   direction.  One block of 64 threads (the reduction uses them all). */
+template <int MODE, int PF>
+__global__ void __launch_bounds__(64) k_topo_line(StepArgs a, TopoArgs t) {
+    __shared__ double rmax[64];
+    __shared__ int rfirst[64];
+    const int I = a.I, J = a.J, K = a.K, IJ = I * J;
+    const size_t N = (size_t)a.N;
+    const double kd = t.kappa * a.scal->dt;
+    const int pfirst = topo_reduce(a, t, rmax, rfirst);
+    const int line = threadIdx.x;
+    if (line >= 2) return;
+    constexpr bool AX = MODE == TOPO_ALONG_X;
+    const int L = AX ? I : K;
+    /* the records of this line, TOPO_LINE_REC doubles per node: the height,
+       the slope to the old + neighbour (the old height of that neighbour at
+       the line's first node), the two denominators and their reciprocals */
+    const double* lrec = (const double*)__builtin_assume_aligned(t.rec, 64) + (size_t)line * L * TOPO_LINE_REC;
+    double* ytop = a.xn + N + (size_t)I * (J - 1) + (AX ? (size_t)IJ * line : (size_t)line);
+    const size_t ystride = AX ? 1 : (size_t)IJ;
+    double ring[PF][6];
+#pragma unroll
+    for (int r = 0; r < PF; r++) {
+        const double* rp = lrec + (size_t)(r < L ? r : L - 1) * TOPO_LINE_REC;
+#pragma unroll
+        for (int q = 0; q < 6; q++) ring[r][q] = rp[q];
+    }
+    /* the body is unconditional (past the line's end it computes on the
+       clamped last record and stores nothing): a branch around the loads
+       makes every node wait for the loads just issued */
+    double yl = 0.0;   /* final height of node s-1 */
+    for (int s0 = 0; s0 < L; s0 += PF) {
+#pragma unroll
+        for (int r = 0; r < PF; r++) {
+            const int s = s0 + r;
+            const double* rc = ring[r];
+            const double y0 = rc[0];
+            const double dy2 = snh_div_by(y0 - yl, rc[2], rc[3]);
+            const double lapl_topo = snh_div_by(rc[1] - dy2, rc[4], rc[5]);
+            const double yi = y0 + kd * lapl_topo;
+            /* AdjustEdgeTopo: the first node copies the old height of its
+               + neighbour, the last the final height of its - neighbour */
+            const double ye = (s == 0) ? rc[1] : yl;
+            const int p = AX ? line * I + s : s * I + line;   /* node order */
+            const bool active = s < L && p >= pfirst;
+            const double y = active ? ((s == 0 || s == L - 1) ? ye : yi) : y0;
+            if (active) ytop[(size_t)s * ystride] = y;
+            yl = y;
+            const double* rp = lrec + (size_t)(s + PF < L ? s + PF : L - 1) * TOPO_LINE_REC;
+#pragma unroll
+            for (int q = 0; q < 6; q++) ring[r][q] = rp[q];
         }
     }
 }
@@ -2700,7 +2782,11 @@ static int launch_node(snh_context* c, int refStep) {
         t.crit = c->cfg.topoGradCriterion;
         hipLaunchKernelGGL(k_topo_prep, dim3(c->topo_parts), dim3(TOPO_PREP_BLOCK), 0, c->stream, a, t);
         HIPCHK(hipGetLastError());
-        if (c->topo_threads <= 256)
+        if (t.mode == TOPO_ALONG_X)
+            hipLaunchKernelGGL((k_topo_line<TOPO_ALONG_X, 16>), dim3(1), dim3(64), 0, c->stream, a, t);
+        else if (t.mode == TOPO_ALONG_Z)
+            hipLaunchKernelGGL((k_topo_line<TOPO_ALONG_Z, 16>), dim3(1), dim3(64), 0, c->stream, a, t);
+        else if (c->topo_threads <= 256)
             hipLaunchKernelGGL((k_topo<256, 8>), dim3(1), dim3(c->topo_threads), 0, c->stream, a, t);
         else
             hipLaunchKernelGGL((k_topo<1024, 3>), dim3(1), dim3(c->topo_threads), 0, c->stream, a, t);
