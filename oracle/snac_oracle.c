@@ -25,7 +25,12 @@
  *                                 five-point stencil's node-order recurrence
  *                                 dh_i = kappa dt (2a + dh_(i-1)/h^2) along x
  *                                 and z, the switch-on node, the edge copies
- *                                 and the running maximum of |grad y|
+ *                                 and the running maximum of |grad y|; on
+ *                                 meshes two nodes wide in x (z) the
+ *                                 three-point recurrence along z (x), the
+ *                                 second row starting from the edge node's
+ *                                 copy of an old height, and no diffusion
+ *                                 two nodes wide in both
  *   elastic                       uniaxial strain closed form
  *   plastic_BI eigen-solver       numpy.linalg.eigh on random symmetric tensors,
  *                                 int-abs truncation cases
