@@ -43,8 +43,8 @@
  *                                 gradient, steady interior), insulated-box
  *                                 energy conservation, 1D diffusion decay rate,
  *                                 thermal stress closed form
- * hypot                         correctly rounded (binary128), checked against
- *                                 exact rational arithmetic; glibc's is selectable
+ * hypot                         the system libm's (glibc 2.35), as the reference
+ *                                 calls it; the correctly rounded value is selectable
  * libm (sin, tan, pow, exp) is the system's glibc, as in the reference's own
  * build.  The device's versions of those four differ from glibc's by an ulp
  * at most, so the GPU paths that call them (softened friction and dilation
@@ -164,14 +164,15 @@ struct orc_context {
 
 static __thread char g_err[256];
 static int g_threads = 1;
-static int g_glibc_hypot = 0;
+static int g_glibc_hypot = 1;
 
-/* hypot of tql2 (plastic_BI/Constitutive.c:430,462).  glibc 2.35's hypot is
-   faithful but not correctly rounded (about 4 in 10^4 results are one ulp
-   off); by default the oracle uses the correctly rounded value, obtained
-   independently in binary128 (the 106-bit squares are exact, the sum and the
-   square root are rounded to 113 bits, then once more to 53).
-   orc_set_glibc_hypot(1) switches to the system libm's hypot. */
+/* hypot of tql2 (plastic_BI/Constitutive.c:430,462): the reference calls the
+   system libm's, here glibc 2.35's, which is faithful but not correctly
+   rounded (a few in 10^3 results are one ulp off).  The oracle calls the same
+   function.  orc_set_glibc_hypot(0) switches to the correctly rounded value,
+   obtained independently in binary128 (the 106-bit squares are exact, the
+   sum and the square root are rounded to 113 bits, then once more to 53), to
+   measure how much the choice matters. */
 static double orc_hypot(double x, double y) {
     if (g_glibc_hypot) return hypot(x, y);
     if (isinf(x) || isinf(y)) return INFINITY;

@@ -105,7 +105,7 @@ int orc_get_tet(orc_context* c, uint32_t e, uint32_t t, orc_tet_view* out);
 int orc_get_element(orc_context* c, uint32_t e, orc_element_view* out);
 int orc_get_node_force(orc_context* c, uint32_t n, double force[3], double* inertialMass);
 
-/* 1 = tql2 uses the system libm hypot; 0 (default) = correctly rounded hypot */
+/* 1 (default) = tql2 uses the system libm hypot, as the reference; 0 = correctly rounded hypot */
 void orc_set_glibc_hypot(int on);
 
 /* eigen-decomposition of plastic_BI (Constitutive.c:240-510), for direct tests */

@@ -29,39 +29,52 @@ SNH_HD int snh_ref_iabs(double x) {
 /* int + int of two abs() results, wrapping as the x86 add does */
 SNH_HD int snh_iadd_wrap(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
 
-/* hypot(x, y) correctly rounded in practice: sqrt of the fused sum of
- * squares, then one residual correction computed exactly with fused
- * multiply-adds (C. F. Borges, "An improved algorithm for hypot(a,b)",
- * arXiv:1904.09481, fused variant).  Power-of-two scaling keeps the squares
- * in range. */
+/* hypot(x, y) as the reference's libm computes it.  The reference calls the
+ * system hypot in tql2 (plastic_BI/Constitutive.c:430,462); on x86-64 with
+ * glibc 2.35 that is the unfused form of C. F. Borges' corrected square root
+ * ("An improved algorithm for hypot(a,b)", arXiv:1904.09481, MyHypot3):
+ * h = sqrt(ax^2 + ay^2) rounded, then one correction h -= (t1 + t2) / 2h
+ * with the error terms of the branch h <= 2 ay or h > 2 ay, evaluated in
+ * plain double arithmetic (no FMA).  Operands below 2^-459 or above 2^511
+ * are scaled by 2^600; a term below 2^-54 of the other is added directly.
+ * The result is faithful, not always correctly rounded (about 2 in 10^3
+ * results are one ulp from the correctly rounded value), and this restatement
+ * gives the system hypot's bits: checked on the host against the libm
+ * function.  Requires FMA contraction off. */
+SNH_HD double snh_hypot_kernel(double ax, double ay) {
+    double h = sqrt(ax * ax + ay * ay);
+    double t1, t2;
+    if (h <= 2.0 * ay) {
+        const double delta = h - ay;
+        t1 = ax * (2.0 * delta - ax);
+        t2 = (delta - 2.0 * (ax - ay)) * delta;
+    } else {
+        const double delta = h - ax;
+        t1 = 2.0 * delta * (ax - 2.0 * ay);
+        t2 = (4.0 * delta - ay) * ay + delta * delta;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return h;
+}
+
 SNH_HD double snh_hypot(double x, double y) {
-    double ax = fabs(x), ay = fabs(y);
-    if (isinf(ax) || isinf(ay)) return INFINITY;
-    if (isnan(ax) || isnan(ay)) return ax + ay;
-    if (ax < ay) {
-        double t = ax;
-        ax = ay;
-        ay = t;
+    if (!isfinite(x) || !isfinite(y)) {
+        if (isinf(x) || isinf(y)) return INFINITY;
+        return x + y;
     }
-    if (ay == 0.0) return ax;
-    /* (ay/ax)^2/2 below 2^-61: the sum of squares rounds to ax^2 */
-    if (ay <= ax * 0x1p-30) return ax;
-    double scale = 1.0;
-    if (ax > 0x1p+500) {
-        ax *= 0x1p-600;
-        ay *= 0x1p-600;
-        scale = 0x1p+600;
-    } else if (ay < 0x1p-500) {
-        ax *= 0x1p+600;
-        ay *= 0x1p+600;
-        scale = 0x1p-600;
+    x = fabs(x);
+    y = fabs(y);
+    const double ax = x < y ? y : x, ay = x < y ? x : y;
+    if (ax > 0x1p+511) {
+        if (ay <= ax * 0x1p-54) return ax + ay;
+        return snh_hypot_kernel(ax * 0x1p-600, ay * 0x1p-600) / 0x1p-600;
     }
-    double h = sqrt(fma(ax, ax, ay * ay));
-    double h_sq = h * h;
-    double ax_sq = ax * ax;
-    double e = fma(-ay, ay, h_sq - ax_sq) + fma(h, h, -h_sq) - fma(ax, ax, -ax_sq);
-    h = h - e / (2.0 * h);
-    return h * scale;
+    if (ay < 0x1p-459) {
+        if (ax >= ay / 0x1p-54) return ax + ay;
+        return snh_hypot_kernel(ax / 0x1p-600, ay / 0x1p-600) * 0x1p-600;
+    }
+    if (ay <= ax * 0x1p-54) return ax + ay;
+    return snh_hypot_kernel(ax, ay);
 }
 
 /* RN(x / 3) without a division.  x = M 2^e with integer M, and ulp(x/3) <=

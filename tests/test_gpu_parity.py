@@ -1,11 +1,15 @@
-"""GPU path vs the CPU oracle on the same inputs (FP64, 1e-12 relative)."""
+"""GPU path vs the CPU oracle on the same inputs, FP64.  Bit-identical
+results are asserted wherever the device runs the same IEEE operations as the
+oracle (every case here but the two that call sin/tan or pow/exp, where the
+device's OCML and glibc may differ by an ulp: those are held to 1e-12 of the
+field's largest magnitude)."""
 import numpy as np
 import pytest
 
 from snac_amd import problem as P
 from tests.parity import FIELDS, max_rel, n_bit_equal, run_both
 
-TOL = 1e-12   # FP64 parity bar: relative to the field's max magnitude
+TOL = 1e-12   # FP64 bar of the library-function cases: relative to the field's max magnitude
 
 pytestmark = pytest.mark.gpu
 
@@ -24,12 +28,20 @@ def _report(name, ref, got, fields=FIELDS):
     return out
 
 
-def _check(name, ref, got, rsc, gsc, fields=FIELDS):
+def _check(name, ref, got, rsc, gsc, fields=FIELDS, exact=True):
+    """exact: every value bit-identical (the claim for paths without library
+    functions); otherwise within TOL of the field's largest magnitude"""
     rep = _report(name, ref, got, fields)
-    for f, (r, _, _) in rep.items():
-        assert r <= TOL, f"{name}: {f} differs by {r:.3e} relative"
-    assert abs(rsc["dt"] - gsc["dt"]) <= TOL * abs(rsc["dt"])
-    assert rsc["minLengthScale"] == pytest.approx(gsc["minLengthScale"], rel=TOL)
+    for f, (r, eq, tot) in rep.items():
+        if exact:
+            assert eq == tot and np.array_equal(ref[f], got[f]), f"{name}: {f} has {tot - eq} differing values"
+        else:
+            assert r <= TOL, f"{name}: {f} differs by {r:.3e} relative"
+    if exact:
+        assert rsc["dt"] == gsc["dt"] and rsc["minLengthScale"] == gsc["minLengthScale"]
+    else:
+        assert abs(rsc["dt"] - gsc["dt"]) <= TOL * abs(rsc["dt"])
+        assert rsc["minLengthScale"] == pytest.approx(gsc["minLengthScale"], rel=TOL)
 
 
 @VARIANTS
@@ -169,7 +181,7 @@ def test_softened_friction_and_dilation_angles(variant):
     m.frictionAngle = (30.0, 20.0, 20.0)
     m.dilationAngle = (10.0, 0.0, 0.0)
     ref, got, rsc, gsc = run_both(p, 40, variant=variant)
-    _check(f"softened_angles[{variant}]", ref, got, rsc, gsc)
+    _check(f"softened_angles[{variant}]", ref, got, rsc, gsc, exact=False)
     ps = ref["plastic_strain"]
     assert np.sum((ps > 0.0) & (ps < 0.05)) > 100   # tets between the table's entries
 
@@ -188,7 +200,7 @@ def test_power_law_arrhenius_maxwell(variant):
     m.activationE = 5.4e3
     fields = FIELDS + ("viscosity",)
     ref, got, rsc, gsc = run_both(p, 10, fields=fields, variant=variant)
-    _check(f"power_law[{variant}]", ref, got, rsc, gsc, fields=fields)
+    _check(f"power_law[{variant}]", ref, got, rsc, gsc, fields=fields, exact=False)
     v = ref["viscosity"]
     assert np.mean((v > m.vis_min) & (v < m.vis_max)) > 0.5
 
