@@ -5,7 +5,7 @@
  * geodynamics/snac tree).  Only tests/, the smoke check and bench.py's CPU
  * baseline use it; the HIP product never does.
  *
- * How each part is pinned (tests/test_oracle_*.py):
+ * How each part is pinned (tests/test_oracle_*.py, tests/test_tangent_model.py):
  *   connectivity, tet tables      exact tables + invariants (5 tets per node per
  *                                 element, face opposite the node, 8E incidences)
  *   tet geometry, minLengthScale  cube closed forms (h^3/6, h^3/3, h^2/2,
@@ -14,13 +14,30 @@
  *                                 testStress-stress (minLengthScale 19.245)
  *   strain rate / rotation        affine velocity fields (exact sym/skew L),
  *                                 rigid motions (zero strain rate)
- *   force, mass                   reference fixtures testUpdateNode-force
- *                                 (-3.31088e9 ... -2.6487e10), -mass (42.1875 ...);
- *                                 force balance with no gravity
+ *   force                         reference fixtures testUpdateNode-force
+ *                                 (-3.31088e9 ... -2.6487e10) and -balance
+ *                                 (accumulation order); force balance without
+ *                                 gravity
+ *   masses                        physical mass: fixture testUpdateNode-mass
+ *                                 (42.1875 ...); dynamic inertial mass
+ *                                 1/4 (lambda+2mu)/c^2 sum V and wave mass
+ *                                 1/4 rho sum V: closed forms on box meshes of
+ *                                 two materials (sum V = abc per incident element);
+ *                                 constant-dt inertial mass: through the
+ *                                 testUpdateNode-velocity fixture, which runs
+ *                                 with dtType constant
  *   node update, BCs              reference fixtures testUpdateNode-velocity
  *                                 (vx = 0.00834447 after 10 iterations) and
  *                                 -coord (0.550852, 1.1)
  *   dt (Sync)                     test-basic timeStep.0 (1.80422e+07)
+ *   element invariant, density    hydrostatic fixture (below); tet density
+ *                                 rho0 (1 - alpha (T - T_ref) + beta sigma_m)
+ *                                 under a uniform stress, and the gravity force
+ *                                 it gives an interior node
+ *   hydrostatic IC                the reference's float32 stress dumps of
+ *                                 basic-hydroStatic.xml, bit for bit; with beta,
+ *                                 second-order convergence to the exponential
+ *                                 profile (1 - exp(-beta rho0 g z)) / beta
  *   topography diffusion          parabolic top surface at rest: the
  *                                 five-point stencil's node-order recurrence
  *                                 dh_i = kappa dt (2a + dh_(i-1)/h^2) along x
@@ -34,16 +51,39 @@
  *   elastic                       uniaxial strain closed form
  *   plastic_BI eigen-solver       numpy.linalg.eigh on random symmetric tensors,
  *                                 int-abs truncation cases
- *   plastic_BI return mapping     yield-surface consistency after a shear return,
- *                                 oedometer-type loading path
+ *   plastic_BI return mapping     oedometer benchmark (Snac/tests/oedometer):
+ *                                 elastic path exact, post-yield path within
+ *                                 3e-5 of analytic_soln.py's corner solution,
+ *                                 plastic strain growing as the measure of the
+ *                                 flow direction times the analytic Epx; shear
+ *                                 return = flow-rule projection with the
+ *                                 multiplier fixed by f_s = 0; tensile return
+ *                                 onto the cut-off (c/tan(phi) or ten_off)
+ *   softening (soften)            per-tet plastic strain increments against
+ *                                 numpy.interp of the tables, on, between and
+ *                                 beyond the nodes (defaults 30 MPa, 30 deg,
+ *                                 0 deg; carry-over between tets)
+ *   plastic-strain increment      recovered from the stress change through the
+ *                                 elastic compliance (shear and tension)
  *   maxwell                       constant-strain relaxation ((1-a)/(1+a))^n
- *   viscoplastic_BI               Maxwell limit below yield; diagonal-only
- *                                 eigen-solve on yield (lower triangle zero)
- *   temperature (energy EPs)      uniform and linear fields (zero flux / exact
- *                                 gradient, steady interior), insulated-box
- *                                 energy conservation, 1D diffusion decay rate,
+ *   viscosity (viscosity_of)      power law: steady stress scales as
+ *                                 rate^(1/n); Arrhenius: relaxation rate at a
+ *                                 uniform temperature
+ *   viscoplastic_BI               Maxwell relaxation below yield; return onto
+ *                                 the surface of the cohesion at the updated
+ *                                 plastic strain (hardening term), where
+ *                                 plastic_BI's is at the previous one;
+ *                                 diagonal-only eigen-solve on yield (lower
+ *                                 triangle zero)
+ *   temperature (energy EPs)      the reference's float32 temperature dumps of
+ *                                 basic-temp.xml, bit for bit; uniform and
+ *                                 linear fields (zero flux / exact gradient,
+ *                                 steady interior), insulated-box energy
+ *                                 conservation, 1D diffusion decay rate,
  *                                 thermal stress closed form
- * hypot                         the system libm's (glibc 2.35), as the reference
+ *   remesh handoff                identity remesh, running minimum of the
+ *                                 length scale, volume jump of a scaled mesh
+ *   hypot                         the system libm's (glibc 2.35), as the reference
  *                                 calls it; the correctly rounded value is selectable
  * libm (sin, tan, pow, exp) is the system's glibc, as in the reference's own
  * build.  The device's versions of those four differ from glibc's by an ulp

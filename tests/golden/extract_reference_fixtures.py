@@ -10,10 +10,18 @@ they come from.  Run in a container that has the reference mounted:
 
 The values are printed by the reference with %g (6 significant digits);
 tests compare at that precision.
+
+It also copies two float32 dump files of the reference's system tests, as
+the data they are (little-endian float32 records, one dump after another):
+  Snac/tests/test-basic-hydroStatic.stress.0.expected  -> hydrostatic_stress.f32
+      element stress scalar of basic-hydroStatic.xml, 12^3 elements, 2 dumps
+  Snac/tests/test-basic-temperature.temperature.0.expected -> temperature.f32
+      nodal temperature of basic-temp.xml, 13^3 nodes, 2 dumps
 """
 import json
 import os
 import re
+import shutil
 
 REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -81,6 +89,9 @@ def main():
     # first dynamic dt of the basic system test (13^3 nodes over 7.5 km, no BCs)
     rel = "Snac/tests/test-basic.0of2.timeStep.0.expected"
     fx["test_basic_dt"] = find(rel, r"^\s*1\s+0\s+(\S+)", lambda m: num(m.group(1)))
+    for src, dst in (("Snac/tests/test-basic-hydroStatic.stress.0.expected", "hydrostatic_stress.f32"),
+                     ("Snac/tests/test-basic-temperature.temperature.0.expected", "temperature.f32")):
+        shutil.copyfile(os.path.join(REF, src), os.path.join(HERE, dst))
     with open(os.path.join(HERE, "reference_fixtures.json"), "w") as f:
         json.dump(fx, f, indent=1)
     print({k: len(v) for k, v in fx.items()})
