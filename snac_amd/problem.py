@@ -399,3 +399,29 @@ WORKLOADS = {
     "mohr_coulomb_weak_crust": mohr_coulomb_weak_crust,
     "maxwell_thermal": maxwell_thermal,
 }
+
+
+def basic_hydrostatic() -> Problem:
+    """Snac/tests/basic-hydroStatic.xml: 13^3 nodes over 7.5 km, density 2700,
+    gravity 9.81, demf 0.8, alpha 0, topo_kappa 0, no velocity conditions,
+    the SnacHydroStaticIC plugin; the other values are the dictionary
+    defaults (lambda = mu = 3e10f, dynamic dt, strain_inert 1e-5f)."""
+    mat = MaterialSpec(alpha=0.0, density=2700.0)
+    p = Problem(elements=(12, 12, 12), extent=((0.0, 7500.0), (-7500.0, 0.0), (0.0, 7500.0)), materials=[mat],
+                gravity=9.81, demf=0.8, topo_kappa=0.0, hydrostatic=True)
+    p.name = "basic_hydrostatic"
+    return p
+
+
+def basic_temperature() -> Problem:
+    """Snac/tests/basic-temp.xml: 13^3 nodes over 7.5 km, density 2700,
+    gravity 0, alpha 0, thermal_conduct 1.6, heatCapacity 0.2, the
+    SnacTemperature plugin with T = 0 and temperature conditions 100 on the
+    top wall, 0 on the bottom; the other values are the dictionary
+    defaults."""
+    mat = MaterialSpec(alpha=0.0, density=2700.0, thermal_conduct=1.6, heatCapacity=0.2)
+    p = Problem(elements=(12, 12, 12), extent=((0.0, 7500.0), (-7500.0, 0.0), (0.0, 7500.0)), materials=[mat],
+                gravity=0.0, demf=0.8, topo_kappa=0.0,
+                thermal=ThermalSpec(temperature=np.zeros(13 ** 3), bcs=[("top", 100.0), ("bottom", 0.0)]))
+    p.name = "basic_temperature"
+    return p

@@ -94,3 +94,21 @@ def test_temperature_dump_and_checkpoint(tmp_path):
     assert (tmp_path / "temperatureCP.0").stat().st_size == 2 * 4 * N
     recs = read_checkpoint(str(tmp_path), p)
     assert np.array_equal(recs[-1]["temperature"], T[2])
+
+
+def test_temperature_dump_file_against_the_reference(tmp_path):
+    """basic-temp.xml run for one step with dumpEvery 1: the temperature file
+    the writer appends (temperature/Output.c:56-75) equals, byte for byte, the
+    record of the reference's test-basic-temperature.temperature.0.expected
+    for that step.  The fixture holds one more record before it, the state
+    at step 0, which its revision also dumped; the current isTimeToDump
+    (Context.c:1534-1541) is false at step 0, and so is the writer's."""
+    import os
+    fx = np.fromfile(os.path.join(os.path.dirname(__file__), "golden", "temperature.f32"), dtype="<f4")
+    p = P.basic_temperature()
+    o = OracleContext(p)
+    w = DumpWriter(str(tmp_path), o, dump_every=1, max_time_steps=1)
+    w.step(1)
+    w.close()
+    got = (tmp_path / "temperature.0").read_bytes()
+    assert got == fx[p.n_nodes:].tobytes()

@@ -64,3 +64,36 @@ def test_dump_files_match_oracle(case, tmp_path):
     E = p.n_elements
     assert (tmp_path / "gpu" / "stress.0").stat().st_size == 3 * 4 * E
     assert (tmp_path / "gpu" / "stressTensorCP.0").stat().st_size == 3 * 240 * E
+
+
+def test_hydrostatic_stress_against_the_reference_dump():
+    """basic-hydroStatic.xml on the GPU: the hydrostatic initial stress and
+    the stress after one step give the reference's float32 'stress' dumps
+    (test-basic-hydroStatic.stress.0.expected) bit for bit, through the
+    invariant TimeStepZero writes (the fixture's revision wrote it every
+    step)."""
+    from tests.test_oracle_pins import old_invariant
+    fx = np.fromfile(os.path.join(os.path.dirname(__file__), "golden", "hydrostatic_stress.f32"),
+                     dtype="<f4").reshape(2, -1)
+    c = Context(P.basic_hydrostatic()).Construct().Build().Initialise()
+    s0 = c.state(fields=("stress",))["stress"]
+    c.Execute(1)
+    s1 = c.state(fields=("stress",))["stress"]
+    c.Destroy()
+    assert np.array_equal(old_invariant(s0).astype(np.float32), fx[0])
+    assert np.array_equal(old_invariant(s1).astype(np.float32), fx[1])
+
+
+def test_temperature_dump_file_against_the_reference(tmp_path):
+    """basic-temp.xml on the GPU, one step, dumpEvery 1: the temperature file
+    equals, byte for byte, the reference's record for that step in
+    test-basic-temperature.temperature.0.expected (the record before it is
+    the step-0 state the fixture's revision also dumped)."""
+    fx = np.fromfile(os.path.join(os.path.dirname(__file__), "golden", "temperature.f32"), dtype="<f4")
+    p = P.basic_temperature()
+    c = Context(p).Construct().Build().Initialise()
+    w = DumpWriter(str(tmp_path), c, dump_every=1, max_time_steps=1)
+    w.step(1)
+    w.close()
+    c.Destroy()
+    assert (tmp_path / "temperature.0").read_bytes() == fx[p.n_nodes:].tobytes()

@@ -469,13 +469,13 @@ def test_tet_density_follows_the_mean_stress(beta, alpha):
 # SnacHydroStaticIC (hydroStaticIC/VariableConditions.c:43-160)
 # --------------------------------------------------------------------------
 
-def _hydro_problem(elements=(12, 12, 12), beta=0.0, depth=7500.0):
+def _hydro_problem(elements=(2, 6, 2), beta=0.0, depth=7500.0):
     mat = P.MaterialSpec(alpha=0.0, beta=beta, density=2700.0)
     return P.Problem(elements=elements, extent=((0.0, 7500.0), (-depth, 0.0), (0.0, 7500.0)), materials=[mat],
                      gravity=9.81, demf=0.8, topo_kappa=0.0, hydrostatic=True)
 
 
-def _old_invariant(stress):
+def old_invariant(stress):
     """0.5 sqrt(0.5 |s11 s22 + s22 s00 + s00 s11 - s01^2 - s12^2 - s02^2|),
     tet mean, of the full stress: what TimeStepZero writes (Context.c:
     1003-1015) and what the fixture's revision of Stress.c wrote every step"""
@@ -496,7 +496,7 @@ def test_hydrostatic_ic_against_the_reference_stress_dump():
     step-1 stress gives them back through the formula TimeStepZero uses.
     Pressure per element row: rho g (depth of the element's mid-height)."""
     fx = np.fromfile(os.path.join(GOLDEN, "hydrostatic_stress.f32"), dtype="<f4").reshape(2, -1)
-    o = Oracle(_hydro_problem())
+    o = Oracle(P.basic_hydrostatic())
     o.hydrostatic_ic()
     o.timestep_zero()
     st0 = o.state()["stress"]
@@ -504,7 +504,7 @@ def test_hydrostatic_ic_against_the_reference_stress_dump():
     assert np.array_equal(zero.astype(np.float32), fx[0])
     o.step(1)
     st1 = o.state()["stress"]
-    assert np.array_equal(_old_invariant(st1).astype(np.float32), fx[1])
+    assert np.array_equal(old_invariant(st1).astype(np.float32), fx[1])
     assert max(o.element(e)["stress"] for e in range(o.E)) == 0.0
     row = (np.arange(o.E) // 12) % 12
     depth = (12 - row - 0.5) * 625.0
@@ -550,11 +550,7 @@ def test_temperature_plugin_against_the_reference_temperature_dump():
     and after the step (test-basic-temperature.temperature.0.expected),
     equal in every float32 bit (the row below the top reaches 13.68534)."""
     fx = np.fromfile(os.path.join(GOLDEN, "temperature.f32"), dtype="<f4").reshape(2, -1)
-    mat = P.MaterialSpec(alpha=0.0, density=2700.0, thermal_conduct=1.6, heatCapacity=0.2)
-    p = P.Problem(elements=(12, 12, 12), extent=((0.0, 7500.0), (-7500.0, 0.0), (0.0, 7500.0)), materials=[mat],
-                  gravity=0.0, demf=0.8, topo_kappa=0.0,
-                  thermal=P.ThermalSpec(temperature=np.zeros(13 ** 3), bcs=[("top", 100.0), ("bottom", 0.0)]))
-    o = Oracle(p)
+    o = Oracle(P.basic_temperature())
     o.timestep_zero()
     assert np.array_equal(o.state()["temperature"].astype(np.float32), fx[0])
     o.step(1)
