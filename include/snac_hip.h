@@ -321,6 +321,20 @@ int snh_autotune(snh_context* ctx, snh_tune_report* out);
 /* Force one element-pass variant (SNH_ELEM_*); both give identical results. */
 int snh_set_elem_variant(snh_context* ctx, int32_t variant);
 
+/* Node-pass variants of the FP64 path (no counterpart in the reference;
+ * both sum every node's tet-face shares in the order of Force.c:87-183 and
+ * give identical results):
+ *   SNH_NODE_PER_NODE  one thread per node, the 8 surrounding elements read
+ *                      from global memory (the default)
+ *   SNH_NODE_TILED     a workgroup per 16 x 8 node columns walking up in k,
+ *                      node coordinates and each element plane's tet volumes
+ *                      staged in LDS, each tet volume computed once per tile
+ * The environment variables SNH_NODE_VARIANT=0|1 and SNH_NODE_KC (node
+ * planes per workgroup of the tiled pass, default 8) set them at creation. */
+#define SNH_NODE_PER_NODE 0
+#define SNH_NODE_TILED    1
+int snh_set_node_variant(snh_context* ctx, int32_t variant);
+
 /* ---- dump diagnostics -----------------------------------------------------
  * While enabled, every step also records what the reference's dumps hold
  * besides the state (Context.c:1311-1489, plastic_BI/Output.c:60-80): the

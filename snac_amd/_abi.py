@@ -145,6 +145,8 @@ ELEM_ELEMENT_PER_LANE = 1
 ELEM_ELEMENT_PER_LANE_LOWREG = 2
 ELEM_TET_PER_WAVE = 3
 ELEM_VARIANTS = 4
+NODE_PER_NODE = 0      # SNH_NODE_* of include/snac_hip.h
+NODE_TILED = 1
 
 
 class TuneReport(C.Structure):
@@ -202,6 +204,7 @@ SIGNATURES = {
     "snh_profile": (_I, [_P, _I]),
     "snh_autotune": (_I, [_P, C.POINTER(TuneReport)]),
     "snh_set_elem_variant": (_I, [_P, C.c_int32]),
+    "snh_set_node_variant": (_I, [_P, C.c_int32]),
     "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
     "snh_set_diagnostics": (_I, [_P, _I]),
     "snh_get_diagnostics": (_I, [_P, C.POINTER(Diagnostics)]),

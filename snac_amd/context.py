@@ -135,6 +135,11 @@ class Context:
         self._check(self.lib.snh_set_elem_variant(self.h, int(variant)))
         return self
 
+    def set_node_variant(self, variant: int):
+        """SNH_NODE_PER_NODE (0) or SNH_NODE_TILED (1): identical results."""
+        self._check(self.lib.snh_set_node_variant(self.h, int(variant)))
+        return self
+
     def set_diagnostics(self, enable: bool = True):
         """Record the reference's dump quantities in the following steps."""
         self._check(self.lib.snh_set_diagnostics(self.h, int(enable)))

@@ -1240,6 +1240,67 @@ __device__ __forceinline__ int gather_node(const StepArgs& a, int g) {
     return i + a.I * (j + a.J * k);
 }
 
+/* the second node pass for node n (UpdateNode.c:60-158) once its force f
+   and inertial mass im are gathered: damping, velocity, velocity BC and
+   coordinates (x: the node's current coordinates); with a replay tape, the
+   undamped force, the mass and the damping branch.  Returns |v| for a
+   constrained node, -inf otherwise (the Sync's vmax, Context.c:1218-1235). */
+__device__ __forceinline__ double node_finish(const StepArgs& a, int n, double f0, double f1, double f2, double im,
+                                              double x0, double x1, double x2, double dt) {
+    double vmag = -INFINITY;
+    const size_t N = (size_t)a.N;
+    double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
+    if (a.tape_f) {
+        /* what the reverse pass needs: the undamped force, the mass and
+           the damping branch of each component */
+        a.tape_f[n] = f0;
+        a.tape_f[N + n] = f1;
+        a.tape_f[2 * N + n] = f2;
+        a.tape_m[n] = im;
+        const int c0 = (int)(GETSIGN(v0) * GETSIGN(f0)) + 1;
+        const int c1 = (int)(GETSIGN(v1) * GETSIGN(f1)) + 1;
+        const int c2 = (int)(GETSIGN(v2) * GETSIGN(f2)) + 1;
+        a.tape_code[n] = (uint8_t)(c0 | (c1 << 2) | (c2 << 4));
+    }
+    double sv0 = GETSIGN(v0), sv1 = GETSIGN(v1), sv2 = GETSIGN(v2);
+    if (a.damp_sched) {
+        const uint8_t ds = a.damp_sched[n];
+        sv0 = (double)((int)(ds & 3u) - 1) * GETSIGN(f0);
+        sv1 = (double)((int)((ds >> 2) & 3u) - 1) * GETSIGN(f1);
+        sv2 = (double)((int)((ds >> 4) & 3u) - 1) * GETSIGN(f2);
+    }
+    f0 -= a.demf * sv0 * fabs(f0);
+    f1 -= a.demf * sv1 * fabs(f1);
+    f2 -= a.demf * sv2 * fabs(f2);
+    if (a.force_out) {
+        a.force_out[n] = f0;
+        a.force_out[N + n] = f1;
+        a.force_out[2 * N + n] = f2;
+    }
+    v0 += dt * f0 / im;
+    v1 += dt * f1 / im;
+    v2 += dt * f2 / im;
+    const uint32_t code = a.bc[n];
+    if (code) {
+        const DevBC& b = a.bcs[code];
+        if (b.mask & 1u) v0 = b.val[0];
+        if (b.mask & 2u) v1 = b.val[1];
+        if (b.mask & 4u) v2 = b.val[2];
+        vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    }
+    /* a NaN or Inf in stress, mass or force reaches the velocities within
+       the step: the counterpart of the reference's release-mode firewall
+       on the element stress (Stress.c:161-166) */
+    if (!(isfinite(v0) && isfinite(v1) && isfinite(v2))) a.scal->nonfinite = 1;
+    a.v_out[n] = v0;
+    a.v_out[N + n] = v1;
+    a.v_out[2 * N + n] = v2;
+    a.xn[n] = x0 + dt * v0;
+    a.xn[N + n] = x1 + dt * v1;
+    a.xn[2 * N + n] = x2 + dt * v2;
+    return vmag;
+}
+
 template <bool MM, bool TH>
 __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
@@ -1305,60 +1366,254 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
                 f1 -= a.gravity * rho * V / 8.0;
             }
         }
-        /* damping, velocity, BC, coordinates (UpdateNode.c:60-158) */
-        const size_t N = (size_t)a.N;
-        double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
-        if (a.tape_f) {
-            /* what the reverse pass needs: the undamped force, the mass and
-               the damping branch of each component */
-            a.tape_f[n] = f0;
-            a.tape_f[N + n] = f1;
-            a.tape_f[2 * N + n] = f2;
-            a.tape_m[n] = im;
-            const int c0 = (int)(GETSIGN(v0) * GETSIGN(f0)) + 1;
-            const int c1 = (int)(GETSIGN(v1) * GETSIGN(f1)) + 1;
-            const int c2 = (int)(GETSIGN(v2) * GETSIGN(f2)) + 1;
-            a.tape_code[n] = (uint8_t)(c0 | (c1 << 2) | (c2 << 4));
-        }
-        double sv0 = GETSIGN(v0), sv1 = GETSIGN(v1), sv2 = GETSIGN(v2);
-        if (a.damp_sched) {
-            const uint8_t ds = a.damp_sched[n];
-            sv0 = (double)((int)(ds & 3u) - 1) * GETSIGN(f0);
-            sv1 = (double)((int)((ds >> 2) & 3u) - 1) * GETSIGN(f1);
-            sv2 = (double)((int)((ds >> 4) & 3u) - 1) * GETSIGN(f2);
-        }
-        f0 -= a.demf * sv0 * fabs(f0);
-        f1 -= a.demf * sv1 * fabs(f1);
-        f2 -= a.demf * sv2 * fabs(f2);
-        if (a.force_out) {
-            a.force_out[n] = f0;
-            a.force_out[N + n] = f1;
-            a.force_out[2 * N + n] = f2;
-        }
-        v0 += dt * f0 / im;
-        v1 += dt * f1 / im;
-        v2 += dt * f2 / im;
-        const uint32_t code = a.bc[n];
-        if (code) {
-            const DevBC& b = a.bcs[code];
-            if (b.mask & 1u) v0 = b.val[0];
-            if (b.mask & 2u) v1 = b.val[1];
-            if (b.mask & 4u) v2 = b.val[2];
-            vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-        }
-        /* a NaN or Inf in stress, mass or force reaches the velocities within
-           the step: the counterpart of the reference's release-mode firewall
-           on the element stress (Stress.c:161-166) */
-        if (!(isfinite(v0) && isfinite(v1) && isfinite(v2))) a.scal->nonfinite = 1;
-        a.v_out[n] = v0;
-        a.v_out[N + n] = v1;
-        a.v_out[2 * N + n] = v2;
-        a.xn[n] = a.xc[n] + dt * v0;
-        a.xn[N + n] = a.xc[N + n] + dt * v1;
-        a.xn[2 * N + n] = a.xc[2 * N + n] + dt * v2;
+        vmag = node_finish(a, n, f0, f1, f2, im, a.xc[n], a.xc[(size_t)a.N + n], a.xc[2 * (size_t)a.N + n], dt);
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+}
+
+/* --------------------------------------------------------------------------
+ * k_node_tile: the same node pass as k_node, organised so that each tet's
+ * volume is computed once per tile instead of once per node of the tet, and
+ * node coordinates come from LDS instead of 24 global loads per element.
+ *
+ * A workgroup owns NT_I x NT_J node columns (i, j) and a chunk of node
+ * planes k0 <= k < k1, and walks up in k.  At element plane k it stages the
+ * coordinates of node plane k+1 (planes k and k+1 sit in a two-slot ring)
+ * and the ten tet volumes of every element of the plane that touches its
+ * columns ((NT_I+1) x (NT_J+1) elements).  Node (i, j, k) then adds its
+ * slots 4..7 (elements (i-1..i, j-1..j, k), dz = 1 in the reference's slot
+ * order s = dx + 2 dy + 4 dz) to the slots 0..3 it took from plane k-1,
+ * and is finished; node (i, j, k+1) takes its slots 0..3 from the same four
+ * elements.  Each node therefore sums its 40 tet-face shares in exactly the
+ * reference's order (Force.c:87-183), and every share is computed with the
+ * same operations as in k_node: the two kernels give the same bits.  A
+ * chunk that starts above plane 0 first takes slots 0..3 of its first node
+ * plane from element plane k0-1.
+ * ------------------------------------------------------------------------ */
+#define NT_I 16
+#define NT_J 8
+#define NT_THREADS (NT_I * NT_J)
+#define NT_CI (NT_I + 2)   /* node columns staged: one halo column each side */
+#define NT_CJ (NT_J + 2)
+#define NT_EI (NT_I + 1)   /* element columns touching the tile's nodes */
+#define NT_EJ (NT_J + 1)
+
+/* one tet face's share in Snac_Force (Force.c:108-173) for the node the face
+   is opposite to: inertial mass, stress . n A / 6, gravity; acc = f0, f1,
+   f2, inertial mass */
+__device__ __forceinline__ void face_share(const StepArgs& a, const DevMaterial& m, double inertialDensity,
+                                           double rho, double V, const double* c1, const double* c2,
+                                           const double* c3, const double st[6], double dt, double acc[4]) {
+    double ax, ay, az;
+    face_cross(c1, c2, c3, ax, ay, az);
+    const double sq = sqrt(ax * ax + ay * ay + az * az);
+    const double area = sq / 2.0;
+    const double ysq = 1.0 / sq;
+    const double n0 = snh_div_by(ax, sq, ysq), n1 = snh_div_by(ay, sq, ysq), n2 = snh_div_by(az, sq, ysq);
+    if (a.dtType == SNH_DT_DYNAMIC) {
+        acc[3] += 0.25 * inertialDensity * V;
+    } else if (a.dtType == SNH_DT_CONSTANT) {
+        double area_sum = 0.0, tmp;
+        tmp = (n0 * area) * (n0 * area);
+        if (tmp > area_sum) area_sum = tmp;
+        tmp = (n1 * area) * (n1 * area);
+        if (tmp > area_sum) area_sum = tmp;
+        tmp = (n2 * area) * (n2 * area);
+        if (tmp > area_sum) area_sum = tmp;
+        acc[3] += m.lam2mu * area_sum / (9.0 * V) * (dt * dt);
+    } else {
+        acc[3] += 0.25 * rho * V;
+    }
+    acc[0] += snh_div6((st[0] * n0 + st[3] * n1 + st[4] * n2) * area);
+    acc[1] += snh_div6((st[3] * n0 + st[1] * n1 + st[5] * n2) * area);
+    acc[2] += snh_div6((st[4] * n0 + st[5] * n1 + st[2] * n2) * area);
+    acc[1] -= a.gravity * rho * V / 8.0;
+}
+
+/* the 40 tet-face shares of a node, per node-slot s and rank r in the
+   reference's order: bits 0-3 the tet (NodeToTetra[s][r]); bits 4+3v..6+3v
+   the hex offset (dx | dy << 1 | dz << 2) of vertex v of the tet face
+   opposite the node (NodeToSurface[s][r], FaceToNode order) */
+struct ShareTable {
+    uint32_t v[8][5];
+};
+constexpr ShareTable make_share_table() {
+    ShareTable tab{};
+    for (int sl = 0; sl < 8; sl++)
+        for (int r = 0; r < 5; r++) {
+            const int t = kNodeToTetra[sl][r], f = kNodeToSurface[sl][r];
+            uint32_t code = (uint32_t)t;
+            for (int v = 0; v < 3; v++) {
+                const int h = kTetraToNode[t][kFaceToNode[f][v]];
+                code |= (uint32_t)(kHexOff[h][0] | (kHexOff[h][1] << 1) | (kHexOff[h][2] << 2)) << (4 + 3 * v);
+            }
+            tab.v[sl][r] = code;
+        }
+    return tab;
+}
+__constant__ ShareTable kShare = make_share_table();
+
+template <bool MM, bool TH>
+__global__ void __launch_bounds__(NT_THREADS) k_node_tile(StepArgs a) {
+    __shared__ double sx[2][3][NT_CJ][NT_CI];     /* node planes k, k+1 (slot = plane & 1) */
+    __shared__ double sv[NTET][NT_EJ][NT_EI];     /* tet volumes of element plane k */
+    __shared__ double sh[NT_THREADS / 64];
+    const int tid = threadIdx.x;
+    const int ti = tid % NT_I, tj = tid / NT_I;
+    const int nti = (a.I + NT_I - 1) / NT_I, ntj = (a.J + NT_J - 1) / NT_J;
+    const int b = xcd_block();
+    const int i0 = (b % nti) * NT_I, j0 = ((b / nti) % ntj) * NT_J;
+    const int k0 = (b / (nti * ntj)) * a.node_kc;
+    const int k1 = min(k0 + a.node_kc, a.K);
+    const int i = i0 + ti, j = j0 + tj;
+    const bool active = i < a.I && j < a.J;
+    const double dt = a.scal->dt;
+    const double c = a.scal->speedOfSound;
+    const size_t E = (size_t)a.E, N = (size_t)a.N;
+
+    auto load_plane = [&](int p) {
+        double(*dst)[NT_CJ][NT_CI] = sx[p & 1];
+        for (int q = tid; q < NT_CI * NT_CJ; q += NT_THREADS) {
+            const int il = q % NT_CI, jl = q / NT_CI;
+            const int gi = i0 - 1 + il, gj = j0 - 1 + jl;
+            double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+            if (gi >= 0 && gi < a.I && gj >= 0 && gj < a.J) {
+                const size_t n = (size_t)gi + (size_t)a.I * ((size_t)gj + (size_t)a.J * p);
+                x0 = a.xc[n];
+                x1 = a.xc[N + n];
+                x2 = a.xc[2 * N + n];
+            }
+            dst[0][jl][il] = x0;
+            dst[1][jl][il] = x1;
+            dst[2][jl][il] = x2;
+        }
+    };
+    auto volumes = [&](int ek) {
+        const int s0 = ek & 1, s1 = s0 ^ 1;
+        for (int q = tid; q < NT_EI * NT_EJ; q += NT_THREADS) {
+            const int el = q % NT_EI, ej = q / NT_EI;
+            const int gi = i0 - 1 + el, gj = j0 - 1 + ej;
+            if (gi < 0 || gi >= a.EI || gj < 0 || gj >= a.EJ) continue;
+            double X[8][3];
+#pragma unroll
+            for (int h = 0; h < 8; h++) {
+                const int sl = kHexOff[h][2] ? s1 : s0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) X[h][d] = sx[sl][d][ej + kHexOff[h][1]][el + kHexOff[h][0]];
+            }
+#pragma unroll
+            for (int t = 0; t < NTET; t++)
+                sv[t][ej][el] = tet_volume(X[kTetraToNode[t][0]], X[kTetraToNode[t][1]], X[kTetraToNode[t][2]],
+                                           X[kTetraToNode[t][3]]);
+        }
+    };
+    /* the five tet-face shares of node-slot `slot` of element e (local
+       lowest node li, lj in the staged planes) into f0..im, in the
+       reference's order; slot and rank are uniform across the workgroup, so
+       the table entry is a scalar load and the LDS offsets scalar values */
+    auto slot_shares = [&](int slot, int e, int li, int lj, int s0, int s1, const DevMaterial& m,
+                           double inertialDensity, double& f0, double& f1, double& f2, double& im) {
+        const double* b0 = &sx[s0][0][lj][li];   /* vertices with dz = 0 (plane ek) */
+        const double* b1 = &sx[s1][0][lj][li];   /* dz = 1 (plane ek + 1) */
+        const double* vb = &sv[0][lj][li];
+        const double* sb = a.stress + e;
+        /* the next share's stress is loaded while this one is computed */
+        uint32_t code = kShare.v[slot][0];
+        double st[6];
+#pragma unroll
+        for (int cc = 0; cc < 6; cc++) st[cc] = sb[((size_t)cc * NTET + (code & 15u)) * E];
+#pragma unroll 1
+        for (int idx = 0; idx < 5; idx++) {
+            const int t = (int)(code & 15u);
+            double stn[6];
+            uint32_t code_n = 0;
+            if (idx < 4) {
+                code_n = kShare.v[slot][idx + 1];
+#pragma unroll
+                for (int cc = 0; cc < 6; cc++) stn[cc] = sb[((size_t)cc * NTET + (code_n & 15u)) * E];
+            }
+            const double rho = TH ? a.dens[(size_t)t * E + e] : m.rho_eff;
+            const double V = vb[t * (NT_EJ * NT_EI)];
+            double p[3][3];
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                const uint32_t o = (code >> (4 + 3 * v)) & 7u;
+                const double* pb = (o & 4u) ? b1 : b0;
+                const int off = (int)((o >> 1) & 1u) * NT_CI + (int)(o & 1u);
+#pragma unroll
+                for (int d = 0; d < 3; d++) p[v][d] = pb[d * (NT_CJ * NT_CI) + off];
+            }
+            double acc[4] = {f0, f1, f2, im};
+            face_share(a, m, inertialDensity, rho, V, p[0], p[1], p[2], st, dt, acc);
+            f0 = acc[0];
+            f1 = acc[1];
+            f2 = acc[2];
+            im = acc[3];
+            if (idx < 4) {
+#pragma unroll
+                for (int cc = 0; cc < 6; cc++) st[cc] = stn[cc];
+                code = code_n;
+            }
+        }
+    };
+    /* from the four elements of plane ek: when finish, slots 4..7 of node
+       (i, j, ek) into fin; when start, slots 0..3 of node (i, j, ek+1) into
+       nxt */
+    auto gather = [&](int ek, bool finish, double fin[4], bool start, double nxt[4]) {
+        const int s0 = ek & 1, s1 = s0 ^ 1;
+#pragma unroll 1
+        for (int sl = 0; sl < 4; sl++) {
+            const int dx = sl & 1, dy = sl >> 1;
+            const int ei = i - 1 + dx, ej = j - 1 + dy;
+            if (ei < 0 || ej < 0 || ei >= a.EI || ej >= a.EJ) continue;
+            const int e = ei + a.EI * (ej + a.EJ * ek);
+            const int li = ti + dx, lj = tj + dy;   /* element's lowest node, in the staged planes */
+            const DevMaterial& m = mat_of<MM>(a, e);
+            const double inertialDensity = m.lam2mu / (c * c);
+            if (finish) slot_shares(sl + 4, e, li, lj, s0, s1, m, inertialDensity, fin[0], fin[1], fin[2], fin[3]);
+            if (start) slot_shares(sl, e, li, lj, s0, s1, m, inertialDensity, nxt[0], nxt[1], nxt[2], nxt[3]);
+        }
+    };
+
+    double vmag = -INFINITY;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};   /* f0, f1, f2, inertial mass of node (i, j, k) */
+    if (k0 >= 1) {
+        load_plane(k0 - 1);
+        load_plane(k0);
+        __syncthreads();
+        volumes(k0 - 1);
+        __syncthreads();
+        if (active) gather(k0 - 1, false, acc, true, acc);   /* slots 0..3 of node plane k0 only */
+        __syncthreads();
+    } else {
+        load_plane(k0);
+        __syncthreads();
+    }
+    for (int k = k0; k < k1; k++) {
+        double nxt[4] = {0.0, 0.0, 0.0, 0.0};
+        const bool up = k < a.K - 1;   /* element plane k exists */
+        const bool start = k + 1 < k1;
+        if (up) {
+            load_plane(k + 1);
+            __syncthreads();
+            volumes(k);
+            __syncthreads();
+            if (active) gather(k, true, acc, start, nxt);
+        }
+        if (active) {
+            const int sl = k & 1;
+            const size_t n = (size_t)i + (size_t)a.I * ((size_t)j + (size_t)a.J * k);
+            const double v = node_finish(a, (int)n, acc[0], acc[1], acc[2], acc[3], sx[sl][0][tj + 1][ti + 1],
+                                         sx[sl][1][tj + 1][ti + 1], sx[sl][2][tj + 1][ti + 1], dt);
+            vmag = fmax(vmag, v);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] = nxt[q];
+        __syncthreads();
+    }
+    const double bm = block_max(vmag, sh);
+    if (tid == 0) a.part_vmax[blockIdx.x] = bm;
 }
 
 /* --------------------------------------------------------------------------
@@ -2085,6 +2340,10 @@ struct snh_context {
     uint32_t timeStep;
     int elem_variant;   /* SNH_ELEM_* */
     int node_band;      /* StepArgs::node_band */
+    int node_variant;   /* SNH_NODE_* */
+    int node_kc;        /* StepArgs::node_kc */
+    int n_tile_blocks;  /* workgroups of k_node_tile */
+    int n_vmax_parts;   /* block maxima of |v| written by the last node pass */
     struct AdjointCtx* adj;   /* recorded adjoint run, or null */
     bool diag_on;             /* steps record dump diagnostics */
     double* diag_buf;         /* [5][E] */
@@ -2246,6 +2505,7 @@ static StepArgs make_args(snh_context* c) {
     a.N = c->N; a.E = c->E;
     a.dtType = c->cfg.dtType;
     a.node_band = c->node_band;
+    a.node_kc = c->node_kc;
     a.gravity = c->cfg.gravity;
     a.demf = c->cfg.demf;
     a.strain_inert = c->cfg.strain_inert;
@@ -2364,7 +2624,12 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     if (c->needs_visc) ALLOC(c->visc, 10 * E * sizeof(double));
     ALLOC(c->bc, N);
     ALLOC(c->part_min, (size_t)c->n_geo_blocks * sizeof(double));
-    ALLOC(c->part_vmax, (size_t)c->n_node_blocks * sizeof(double));
+    c->node_kc = 8;
+    if (const char* kc = getenv("SNH_NODE_KC")) c->node_kc = atoi(kc) > 0 ? atoi(kc) : 8;
+    c->n_tile_blocks = ((c->I + NT_I - 1) / NT_I) * ((c->J + NT_J - 1) / NT_J) * ((c->K + c->node_kc - 1) / c->node_kc);
+    c->n_vmax_parts = c->n_node_blocks;
+    ALLOC(c->part_vmax, (size_t)(c->n_node_blocks > c->n_tile_blocks ? c->n_node_blocks : c->n_tile_blocks) *
+                            sizeof(double));
     ALLOC(c->scal, sizeof(DevScalars));
     ALLOC(c->dmat, SNH_MAX_MATERIALS * sizeof(DevMaterial));
     ALLOC(c->dbc, 256 * sizeof(DevBC));
@@ -2480,6 +2745,8 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     c->cur = 0;
     c->elem_variant = SNH_ELEM_TET_PER_LANE;
     c->node_band = 8;
+    c->node_variant = SNH_NODE_PER_NODE;
+    if (const char* nv = getenv("SNH_NODE_VARIANT")) c->node_variant = atoi(nv) == SNH_NODE_TILED ? SNH_NODE_TILED : SNH_NODE_PER_NODE;
     if (const char* nb = getenv("SNH_NODE_BAND")) c->node_band = atoi(nb);
     c->adj = nullptr;
     c->diag_on = false;
@@ -2645,7 +2912,7 @@ static int launch_geo(snh_context* c) {
 static int launch_fin(snh_context* c) {
     KTimer kt(c, KK_FIN);
     StepArgs a = make_args(c);
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, a, c->n_geo_blocks, c->n_node_blocks);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, a, c->n_geo_blocks, c->n_vmax_parts);
     HIPCHK(hipGetLastError());
     return SNH_OK;
 }
@@ -2753,22 +3020,38 @@ static int sched_index(const snh_context* c, uint32_t t) {
     return (int)(t - c->sched_base);
 }
 
+/* the FP64 node pass in the context's variant; records how many block
+   maxima of |v| it leaves for k_fin */
+static void launch_node_fp64(snh_context* c, const StepArgs& a) {
+    if (c->node_variant == SNH_NODE_TILED) {
+        const dim3 g(c->n_tile_blocks), b(NT_THREADS);
+        if (a.dens && a.mat) hipLaunchKernelGGL((k_node_tile<true, true>), g, b, 0, c->stream, a);
+        else if (a.dens) hipLaunchKernelGGL((k_node_tile<false, true>), g, b, 0, c->stream, a);
+        else if (a.mat) hipLaunchKernelGGL((k_node_tile<true, false>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_node_tile<false, false>), g, b, 0, c->stream, a);
+        c->n_vmax_parts = c->n_tile_blocks;
+        return;
+    }
+    const dim3 g(c->n_node_blocks), b(NODE_BLOCK);
+    if (a.dens && a.mat) hipLaunchKernelGGL((k_node<true, true>), g, b, 0, c->stream, a);
+    else if (a.dens) hipLaunchKernelGGL((k_node<false, true>), g, b, 0, c->stream, a);
+    else if (a.mat) hipLaunchKernelGGL((k_node<true, false>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_node<false, false>), g, b, 0, c->stream, a);
+    c->n_vmax_parts = c->n_node_blocks;
+}
+
 /* refStep: the reference's timeStep of the step being run (1 for the first) */
 static int launch_node(snh_context* c, int refStep) {
     KTimer kt(c, KK_NODE);
     StepArgs a = make_args(c);
     const int si = sched_index(c, c->timeStep);
     if (si >= 0 && c->sched_damp) a.damp_sched = c->sched_damp + (size_t)si * c->N;
-    if (c->cfg.precision == SNH_PRECISION_MIXED)
+    if (c->cfg.precision == SNH_PRECISION_MIXED) {
         hipLaunchKernelGGL(k_node_mx, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
-    else if (a.dens && a.mat)
-        hipLaunchKernelGGL((k_node<true, true>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
-    else if (a.dens)
-        hipLaunchKernelGGL((k_node<false, true>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
-    else if (a.mat)
-        hipLaunchKernelGGL((k_node<true, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_node<false, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+        c->n_vmax_parts = c->n_node_blocks;
+    } else {
+        launch_node_fp64(c, a);
+    }
     HIPCHK(hipGetLastError());
     if (c->cfg.topo_kappa > 0 && refStep > 1 && topo_mode(c->I, c->K) != TOPO_NONE) {
         /* before the buffer flip: xc still holds the old coordinates */
@@ -3002,6 +3285,7 @@ extern "C" int snh_timestep_zero(snh_context* c) {
                           hipMemcpyHostToDevice, c->stream));
     StepArgs a = make_args(c);
     hipLaunchKernelGGL(k_vmax, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
+    c->n_vmax_parts = c->n_node_blocks;
     HIPCHK(hipGetLastError());
     if ((rc = launch_geo(c))) return rc;
     if ((rc = launch_fin(c))) return rc;
@@ -3341,6 +3625,13 @@ extern "C" int snh_set_elem_variant(snh_context* c, int32_t v) {
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (v < 0 || v >= SNH_ELEM_VARIANTS) return set_err(SNH_EINVAL, "unknown element variant");
     c->elem_variant = v;
+    return SNH_OK;
+}
+
+extern "C" int snh_set_node_variant(snh_context* c, int32_t v) {
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (v != SNH_NODE_PER_NODE && v != SNH_NODE_TILED) return set_err(SNH_EINVAL, "unknown node variant");
+    c->node_variant = v;
     return SNH_OK;
 }
 

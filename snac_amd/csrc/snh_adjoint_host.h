@@ -172,14 +172,11 @@ static int adj_run_tape(snh_context* c, AdjointCtx* A, uint32_t t0, int i0, int 
         an.tape_m = A->tm + (size_t)i * N;
         an.tape_code = A->tc + (size_t)i * N;
         if (si >= 0 && c->sched_damp) an.damp_sched = c->sched_damp + (size_t)si * N;
-        if (an.mat)
-            hipLaunchKernelGGL((k_node<true, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, an);
-        else
-            hipLaunchKernelGGL((k_node<false, false>), dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, an);
+        launch_node_fp64(c, an);
         StepArgs ag = a;
         ag.xc = a.xn;
         hipLaunchKernelGGL(k_geo, dim3(c->n_geo_blocks), dim3(GEO_BLOCK), 0, c->stream, ag);
-        hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, ag, c->n_geo_blocks, c->n_node_blocks);
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, ag, c->n_geo_blocks, c->n_vmax_parts);
         if (hipGetLastError() != hipSuccess) rc = set_err(SNH_EHIP, "adjoint tape step launch failed");
     }
     return rc;

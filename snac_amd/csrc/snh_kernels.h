@@ -136,6 +136,7 @@ struct StepArgs {
     int32_t N, E;
     int32_t dtType;
     int32_t node_band;      /* k-planes per band of the node passes' order, 0 = plain */
+    int32_t node_kc;        /* node planes per workgroup of the tiled node pass */
     double gravity, demf, strain_inert;
     double c_wave;          /* the wave dt type's speed of sound: max Vp over the elements */
 };
