@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SNH_ABI_VERSION 3
+#define SNH_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------- */
 #define SNH_OK            0
@@ -439,6 +439,29 @@ int snh_ensemble_create(snh_context* const* members, int32_t n, int32_t graph_st
 int snh_ensemble_step(snh_ensemble* ens, int32_t nsteps);
 int snh_ensemble_synchronize(snh_ensemble* ens);
 int snh_ensemble_destroy(snh_ensemble* ens);
+
+/* ---- allocation-free Sync -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the Sync of a mesh state held in the caller's device
+ * memory, with no allocation and no context.  It computes what
+ * snh_timestep_zero computes for a context holding that state: the minimum
+ * length scale, min V/A over every tet face (UpdateElement.c:108-139), the
+ * largest |v| over the nodes on the walls of cfg's velocity conditions,
+ * floored at 3e-11f, and the dt of cfg->dtType with its speed of sound
+ * (_Snac_Context_Sync, Context.c:1208-1268; the constant type keeps
+ * cfg->timeStep; the wave type takes the largest Vp over cfg's materials).
+ *   coord, velocity  device memory, component rows: x[N] y[N] z[N], node
+ *                    n = i + I (j + J k) of cfg's mesh
+ *   out              device memory, 4 doubles: dt, speedOfSound,
+ *                    minLengthScale, vmax
+ *   stream           hipStream_t as void*, NULL for the default stream
+ * Enqueues its kernels and returns without waiting.  The block partials and
+ * the scalars live in one fixed device array of the library, 393 KB on each
+ * GPU it runs on, held until the process ends: meshes of at most 4194304
+ * nodes and 4194304 elements (SNH_EINVAL beyond).  Every call on a GPU uses
+ * that GPU's array, so calls that overlap on one GPU race. */
+int snh_sync_state(const snh_config* cfg, const double* coord, const double* velocity, double* out, void* stream);
 
 /* Device-to-device copy bandwidth of the current GPU (GB/s, read + write
  * bytes), from a grid-stride copy of nbytes timed with HIP events over iters

@@ -205,6 +205,7 @@ SIGNATURES = {
     "snh_autotune": (_I, [_P, C.POINTER(TuneReport)]),
     "snh_set_elem_variant": (_I, [_P, C.c_int32]),
     "snh_set_node_variant": (_I, [_P, C.c_int32]),
+    "snh_sync_state": (_I, [C.POINTER(Config), _P, _P, _P, _P]),
     "snh_measure_copy_bandwidth": (_I, [C.c_size_t, _I, C.POINTER(C.c_double)]),
     "snh_set_diagnostics": (_I, [_P, _I]),
     "snh_get_diagnostics": (_I, [_P, C.POINTER(Diagnostics)]),
@@ -222,7 +223,7 @@ SIGNATURES = {
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 
-ABI_VERSION = 3          # SNH_ABI_VERSION of include/snac_hip.h
+ABI_VERSION = 4          # SNH_ABI_VERSION of include/snac_hip.h
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libsnac_hip.so")

@@ -27,6 +27,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3701,6 +3702,145 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
     if (getenv("SNH_ELEM_VARIANT")) c->elem_variant = saved;
     rep.elem_variant = c->elem_variant;
     if (out) *out = rep;
+    return SNH_OK;
+}
+
+/* ==========================================================================
+ * snh_sync_state: the Sync of a state in caller device memory, allocating
+ * nothing.  The block partials of the minimum length scale and of max |v|
+ * and the step scalars live in one fixed device array (`scratch`), whose
+ * device address each GPU resolves once with hipGetSymbolAddress.
+ * ======================================================================== */
+#define SYNC_MAX_ELEMENTS (1 << 22)
+#define SYNC_MAX_NODES (1 << 22)
+#define SYNC_GEO_PARTS (SYNC_MAX_ELEMENTS / GEO_BLOCK)    /* k_geo block minima */
+#define SYNC_NODE_PARTS (SYNC_MAX_NODES / NODE_BLOCK)     /* k_vmax_walls block maxima */
+#define SYNC_SCAL_DOUBLES 16                              /* DevScalars, rounded up */
+static_assert(sizeof(DevScalars) <= SYNC_SCAL_DOUBLES * sizeof(double), "scalars do not fit the scratch");
+static __device__ __align__(256) double scratch[SYNC_GEO_PARTS + SYNC_NODE_PARTS + SYNC_SCAL_DOUBLES];
+
+#define SYNC_MAX_DEVICES 64
+static double* g_scratch_ptr[SYNC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* scratch's address on the current device, resolved on the first call there */
+static int sync_scratch(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SYNC_MAX_DEVICES) return set_err(SNH_EUNSUPPORTED, "device %d beyond %d", dev, SYNC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double* p = nullptr;
+        hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(SNH_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SNH_OK;
+}
+
+/* the scalars as snh_context_create sets them: dt = the configured timeStep */
+__global__ void k_sync_init(DevScalars* s, double dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s->dt = dt;
+        s->dt_rot = dt;
+        s->speedOfSound = 0.0;
+        s->minLen = 0.0;
+        s->vmax = 0.0;
+        s->thermal_err = 0;
+        s->nonfinite = 0;
+        s->topoGradMax = 0.0;
+    }
+}
+
+/* block maxima of |v| over the nodes on the walls of the velocity
+   conditions (bit w of `walls` for SNH_WALL_w), as k_vmax over the
+   context's BC codes */
+__global__ void __launch_bounds__(NODE_BLOCK) k_vmax_walls(StepArgs a, uint32_t walls) {
+    __shared__ double sh[NODE_BLOCK / 64];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double vmag = -INFINITY;
+    if (n < a.N) {
+        const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+        const bool on = ((walls >> SNH_WALL_ALL) & 1u) || (((walls >> SNH_WALL_FRONT) & 1u) && a.K > 1 && k == 0) ||
+                        (((walls >> SNH_WALL_BACK) & 1u) && a.K > 1 && k == a.K - 1) ||
+                        (((walls >> SNH_WALL_LEFT) & 1u) && a.I > 1 && i == 0) ||
+                        (((walls >> SNH_WALL_RIGHT) & 1u) && a.I > 1 && i == a.I - 1) ||
+                        (((walls >> SNH_WALL_TOP) & 1u) && a.J > 1 && j == a.J - 1) ||
+                        (((walls >> SNH_WALL_BOTTOM) & 1u) && a.J > 1 && j == 0);
+        if (on) {
+            const size_t N = (size_t)a.N;
+            const double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
+            vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        }
+    }
+    const double bm = block_max(vmag, sh);
+    if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+}
+
+__global__ void k_sync_out(const DevScalars* s, double* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        out[0] = s->dt;
+        out[1] = s->speedOfSound;
+        out[2] = s->minLen;
+        out[3] = s->vmax;
+    }
+}
+
+extern "C" int snh_sync_state(const snh_config* cfg, const double* coord, const double* velocity, double* out,
+                              void* stream) {
+    Range range_("snh_sync_state");
+    if (!cfg) return set_err(SNH_EINVAL, "snh_sync_state: null config");
+    if (cfg->meshSizeI < 2 || cfg->meshSizeJ < 2 || cfg->meshSizeK < 2)
+        return set_err(SNH_EINVAL, "mesh needs at least 2 nodes per axis");
+    const size_t I = cfg->meshSizeI, J = cfg->meshSizeJ, K = cfg->meshSizeK;
+    const size_t N = I * J * K, E = (I - 1) * (J - 1) * (K - 1);
+    if (N > SYNC_MAX_NODES || E > SYNC_MAX_ELEMENTS)
+        return set_err(SNH_EINVAL, "snh_sync_state: %zu nodes / %zu elements beyond its %d / %d", N, E,
+                       SYNC_MAX_NODES, SYNC_MAX_ELEMENTS);
+    if (cfg->dtType != SNH_DT_DYNAMIC && cfg->dtType != SNH_DT_CONSTANT && cfg->dtType != SNH_DT_WAVE)
+        return set_err(SNH_EINVAL, "bad dtType");
+    if (cfg->dtType == SNH_DT_WAVE && (cfg->nmaterials < 1 || !cfg->materials))
+        return set_err(SNH_EINVAL, "the wave dt type needs the materials");
+    if (cfg->nbcs && !cfg->bcs) return set_err(SNH_EINVAL, "nbcs > 0 with null bcs");
+    if (!coord || !velocity || !out) return set_err(SNH_EINVAL, "snh_sync_state: null device pointer");
+    uint32_t walls = 0;
+    for (uint32_t b = 0; b < cfg->nbcs; b++) {
+        if (cfg->bcs[b].wall < SNH_WALL_FRONT || cfg->bcs[b].wall > SNH_WALL_ALL)
+            return set_err(SNH_EINVAL, "bad wall %d", cfg->bcs[b].wall);
+        walls |= 1u << cfg->bcs[b].wall;
+    }
+    double* sp = nullptr;
+    int rc = sync_scratch(&sp);
+    if (rc) return rc;
+    StepArgs a;
+    memset(&a, 0, sizeof a);
+    a.xc = coord;
+    a.v = velocity;
+    a.part_min = sp;
+    a.part_vmax = sp + SYNC_GEO_PARTS;
+    a.scal = (DevScalars*)(sp + SYNC_GEO_PARTS + SYNC_NODE_PARTS);
+    a.I = (int)I;
+    a.J = (int)J;
+    a.K = (int)K;
+    a.EI = (int)I - 1;
+    a.EJ = (int)J - 1;
+    a.EK = (int)K - 1;
+    a.N = (int)N;
+    a.E = (int)E;
+    a.dtType = cfg->dtType;
+    a.strain_inert = cfg->strain_inert;
+    a.c_wave = 0.0;
+    for (uint32_t q = 0; q < cfg->nmaterials && cfg->materials; q++) a.c_wave = wave_speed_max(a.c_wave, cfg->materials[q]);
+    hipStream_t st = (hipStream_t)stream;
+    const int ngeo = (int)((E + GEO_BLOCK - 1) / GEO_BLOCK), nnode = (int)((N + NODE_BLOCK - 1) / NODE_BLOCK);
+    hipLaunchKernelGGL(k_sync_init, dim3(1), dim3(64), 0, st, a.scal, cfg->timeStep);
+    hipLaunchKernelGGL(k_vmax_walls, dim3(nnode), dim3(NODE_BLOCK), 0, st, a, walls);
+    hipLaunchKernelGGL(k_geo, dim3(ngeo), dim3(GEO_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(FIN_BLOCK), 0, st, a, ngeo, nnode);
+    hipLaunchKernelGGL(k_sync_out, dim3(1), dim3(64), 0, st, (const DevScalars*)a.scal, out);
+    HIPCHK(hipGetLastError());
     return SNH_OK;
 }
 

@@ -61,3 +61,13 @@ def test_struct_sizes_match_c():
     assert sizes == [C.sizeof(_abi.Material), C.sizeof(_abi.VelocityBC), C.sizeof(_abi.Config),
                      C.sizeof(_abi.State), C.sizeof(_abi.Scalars), C.sizeof(_abi.KernelTimes),
                      C.sizeof(_abi.TemperatureBC), C.sizeof(_abi.Thermal)]
+
+
+def test_sync_state_checks_sizes_before_touching_the_gpu():
+    """snh_sync_state refuses a mesh beyond its fixed scratch before any HIP
+    call, so this runs without a GPU."""
+    from snac_amd import problem as P
+    lib = _abi.load_library()
+    cfg, keep = P.elastic_box_compression(elements=(256, 128, 129)).to_c()   # 4.2 M elements
+    rc = lib.snh_sync_state(C.byref(cfg), C.c_void_p(1), C.c_void_p(1), C.c_void_p(1), None)
+    assert rc == _abi.SNH_EINVAL and b"beyond" in lib.snh_last_error()
