@@ -226,6 +226,145 @@ def ensemble_adjoint_bench(probs, steps, interval=10):
             "finite_gradients": all(bool(np.all(np.isfinite(g))) for _, g in res)}
 
 
+def mixed_c3_bench(steps=200, warmup=10):
+    """The mixed-precision workload of the metric: Maxwell viscoelastic with
+    a log-normal viscosity field, 192 x 48 x 192 elements over 300 x 75 x
+    300 km.  FP64 and FP32-bulk/FP64-state runs of the same inputs, each
+    autotuned, warmed up and timed over `steps` steps; the error is the
+    largest top-surface uplift difference relative to the largest uplift
+    after those steps, and likewise for stress and velocity."""
+    import numpy as np
+    from snac_amd import Context, _abi
+    from snac_amd import problem as P
+    from snac_amd.autograd import top_nodes
+    res = {}
+    for prec in ("fp64", "mixed"):
+        p = P.maxwell_viscoelastic()
+        p.precision = _abi.PRECISION_MIXED if prec == "mixed" else _abi.PRECISION_FP64
+        c = Context(p).Construct().Build().Initialise()
+        x0 = c.state(fields=("coord",))["coord"]
+        tune = c.autotune()
+        c.step(warmup)
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.synchronize()
+        dt = time.perf_counter() - t0
+        st = c.state(fields=("coord", "velocity", "stress"))
+        c.Destroy()
+        top = top_nodes(p)
+        res[prec] = {"rate": p.n_elements * steps / dt, "ms_per_step": 1e3 * dt / steps,
+                     "uplift": st["coord"][top, 1] - x0[top, 1], "st": st, "variant": tune["elem_variant"]}
+    a, b = res["fp64"], res["mixed"]
+    rel = lambda x, y: float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+    return {"workload": "maxwell_viscoelastic", "elements": [192, 48, 192], "steps": steps + warmup,
+            "timed_steps": steps, "fp64_value": a["rate"], "mixed_value": b["rate"],
+            "unit": "element-steps/s", "mixed_vs_fp64": b["rate"] / a["rate"],
+            "fp64_ms_per_step": a["ms_per_step"], "mixed_ms_per_step": b["ms_per_step"],
+            "error_vs_fp64": {"topography": rel(b["uplift"], a["uplift"]),
+                              "stress": rel(b["st"]["stress"], a["st"]["stress"]),
+                              "velocity": rel(b["st"]["velocity"], a["st"]["velocity"])},
+            "elem_variants": {"fp64": a["variant"], "mixed": b["variant"]}}
+
+
+def _pmc_child_cmd(args):
+    cmd = [sys.executable, os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--no-cpu", "--no-adjoint",
+           "--no-ensemble", "--no-traffic", "--no-mixed-c3", "--workload", args.workload, "--precision", args.precision]
+    if args.elements:
+        cmd += ["--elements", args.elements]
+    return cmd
+
+
+def _pmc_run(counters, args):
+    """Per-dispatch counter values {kernel name: [value per dispatch]} of a
+    short bench (3 steps) under rocprofv3 --pmc, or None."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        sys.stderr.write("bench.py: rocprofv3 not found; roofline.traffic left empty\n")
+        return None
+    d = tempfile.mkdtemp(prefix="snac_pmc_")
+    cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc"] + list(counters) + ["-d", d, "-o", "pmc", "-f", "csv",
+                                                                             "--"] + _pmc_child_cmd(args)
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=170)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            sys.stderr.write(f"bench.py: rocprofv3 {counters} gave no counters (rc {r.returncode})\n")
+            return None
+        per = {}
+        for fn in files:
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    key = (row["Dispatch_Id"], row["Kernel_Name"])
+                    per.setdefault(key, {}).setdefault(row["Counter_Name"], 0.0)
+                    per[key][row["Counter_Name"]] += float(row["Counter_Value"])
+        out = {}
+        for (_, name), vals in per.items():
+            out.setdefault(name, []).append(vals)
+        return out
+    except (OSError, subprocess.SubprocessError, KeyError, ValueError) as exc:
+        sys.stderr.write(f"bench.py: rocprofv3 {counters} failed: {exc}\n")
+        return None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def _kernel_rows(res, prefix):
+    """dispatches of the most-dispatched kernel whose name starts with prefix
+    (the variant the timed steps ran; the autotuner launches the others a
+    few times each)"""
+    names = [n for n in res if n.split("(")[0].split("<")[0].strip().endswith(prefix)]
+    if not names:
+        return None, None
+    name = max(names, key=lambda n: len(res[n]))
+    return name, res[name]
+
+
+def hbm_traffic(args, kernel_prefixes=("k_elem", "k_node", "k_geo")):
+    """HBM bytes per launch of the step's kernels, from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for
+    the copy probe k_copy of known size (1 GiB read, 1 GiB written per
+    launch), and the VALU issue fraction from SQ_INSTS_VALU against the
+    chip's 256 wave-instructions per cycle over the kernel's cycles
+    (GRBM_GUI_ACTIVE summed over the 8 XCDs)."""
+    out = {}
+    res = {c: _pmc_run([c], args) for c in ("FETCH_SIZE", "WRITE_SIZE")}
+    if not all(res.values()):
+        return None
+    cal = {}
+    for c in res:
+        _, rows = _kernel_rows(res[c], "k_copy")
+        if not rows:
+            sys.stderr.write(f"bench.py: no k_copy dispatch among {sorted(res[c])[:8]}\n")
+            return None
+        cal[c] = (1 << 30) / (1024.0 * sum(r[c] for r in rows) / len(rows))
+    valu = _pmc_run(["SQ_INSTS_VALU", "GRBM_GUI_ACTIVE"], args)
+    for pre in kernel_prefixes:
+        name_f, rf = _kernel_rows(res["FETCH_SIZE"], pre)
+        name_w, rw = _kernel_rows(res["WRITE_SIZE"], pre)
+        if not rf or not rw:
+            continue
+        fetch = 1024.0 * sum(r["FETCH_SIZE"] for r in rf) / len(rf) * cal["FETCH_SIZE"]
+        write = 1024.0 * sum(r["WRITE_SIZE"] for r in rw) / len(rw) * cal["WRITE_SIZE"]
+        k = {"kernel": name_f, "read_bytes": fetch, "write_bytes": write, "bytes": fetch + write}
+        if valu:
+            name_v, rv = _kernel_rows(valu, pre)
+            if rv:
+                insts = sum(r["SQ_INSTS_VALU"] for r in rv) / len(rv)
+                cycles = sum(r["GRBM_GUI_ACTIVE"] for r in rv) / len(rv) / 8.0
+                k["valu_issue_frac"] = insts / (256.0 * cycles) if cycles > 0 else None
+        out[pre] = k
+    out["calibration"] = {"fetch_bytes_per_kib": 1024.0 * cal["FETCH_SIZE"],
+                          "write_bytes_per_kib": 1024.0 * cal["WRITE_SIZE"],
+                          "probe": "k_copy, 1 GiB read + 1 GiB written per launch"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,12 +382,18 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-model ensemble line")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs behind roofline.traffic")
+    ap.add_argument("--no-mixed-c3", action="store_true", help="skip the mixed-precision Maxwell line")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
               "(sharding over several GPUs is designed, not built).", file=sys.stderr)
         sys.exit(2)
+
+    # counter runs first, in child processes, before this process touches the GPU
+    traffic = None if args.no_traffic else hbm_traffic(args)
 
     from snac_amd import Context, _abi, metrics
     problem = make_problem(args.workload, args.elements)
@@ -324,7 +469,7 @@ def main():
             "peak": metrics.HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / metrics.HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": (traffic or {}).get("k_" + dom, {}).get("bytes"),
             "bytes_per_launch": dom_bytes,
             "avg_launch_ms": per[dom],
             "measured_copy_peak": measured_peak,
@@ -345,6 +490,10 @@ def main():
         },
         "autotune": tune,
     }
+    if traffic and "k_" + dom in traffic:
+        out["roofline"]["valu_issue_frac"] = traffic["k_" + dom].get("valu_issue_frac")
+    if traffic:
+        out["counters"] = traffic
 
     if not args.no_adjoint and args.precision == "fp64":
         from snac_amd import problem as P
@@ -354,6 +503,9 @@ def main():
             adjoint_bench(lambda: P.mohr_coulomb_shear(elements=(64, 32, 64)), _abi.PARAM_COHESION,
                           args.adjoint_steps, 50, _abi.ADJ_CORNER_CLAMP),
         ]
+
+    if not args.no_mixed_c3 and args.precision == "fp64":
+        out["mixed_c3"] = mixed_c3_bench()
 
     if not args.no_ensemble and args.precision == "fp64":
         out["ensemble"] = ensemble_bench(64, args.ensemble_steps)

@@ -186,7 +186,11 @@ int snh_context_destroy(snh_context* ctx);
  * context's own stream. */
 int snh_set_stream(snh_context* ctx, void* stream);
 
-/* Per-element material index (element->material_I).  nelem entries. */
+/* Per-element material index (element->material_I).  nelem entries.  For
+ * the wave dt type the speed of sound is the largest Vp over the materials
+ * in use; after snh_timestep_zero it is a running maximum, as the
+ * reference's (Context.c:392, UpdateElement.c:62-67): a later call can
+ * raise it, never lower it. */
 int snh_set_element_material(snh_context* ctx, const uint8_t* material, size_t nelem);
 
 /* Per-element input field (SNH_FIELD_*), nelem doubles. */

@@ -56,16 +56,28 @@ class SurfaceUplift(torch.autograd.Function):
         c = Context(p).Construct().Build().Initialise()
         top = top_nodes(p)
         y0 = c.state(fields=("coord",))["coord"][top, 1]
+        ctx.top = top
+        ctx.snac = None
+        if not ctx.needs_input_grad[0]:
+            # no gradient wanted: plain steps, no checkpoints or tape kept
+            try:
+                c.Execute(nsteps)
+                y = c.state(fields=("coord",))["coord"][top, 1]
+            finally:
+                c.Destroy()
+            return torch.as_tensor(y - y0, dtype=theta.dtype, device=theta.device)
         c.adjoint_begin(param, checkpoint_interval, flags)
         c.adjoint_forward(nsteps)
         y = c.state(fields=("coord",))["coord"][top, 1]
         ctx.snac = c
-        ctx.top = top
         return torch.as_tensor(y - y0, dtype=theta.dtype, device=theta.device)
 
     @staticmethod
     def backward(ctx, grad_uplift):
         c = ctx.snac
+        if c is None:
+            raise RuntimeError("surface_uplift: the adjoint record is released after the first backward; "
+                               "run the forward again for another backward")
         cb = np.zeros((c.N, 3))
         cb[ctx.top, 1] = grad_uplift.detach().cpu().numpy()
         try:

@@ -130,6 +130,12 @@ class Context:
                 "elem_ms_element_per_lane_lowreg": r.elem_ms[2], "elem_ms_tet_per_wave": r.elem_ms[3],
                 "elem_variant": r.elem_variant}
 
+    def set_element_material(self, material):
+        """Per-element material index, [E] (snh_set_element_material)."""
+        m = np.ascontiguousarray(material, dtype=np.uint8)
+        self._check(self.lib.snh_set_element_material(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8)), m.size))
+        return self
+
     def set_elem_variant(self, variant: int):
         """Force the element-pass variant (tet per lane or element per lane)."""
         self._check(self.lib.snh_set_elem_variant(self.h, int(variant)))

@@ -2538,10 +2538,14 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     if (cfg->nmaterials < 1 || cfg->nmaterials > SNH_MAX_MATERIALS || !cfg->materials)
         return set_err(SNH_EINVAL, "need 1..%d materials", SNH_MAX_MATERIALS);
     if (cfg->topo_kappa > 0) {
-        if (cfg->meshSizeI > TOPO_MAX_I)
-            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than %d nodes along x", TOPO_MAX_I);
-        if (cfg->meshSizeI > 1024 && cfg->meshSizeK > 1024)
-            return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than 1024 nodes along both x and z");
+        /* the surface sweep keeps two rows of heights in LDS; the
+           one-dimensional modes keep none */
+        if (topo_mode((int)cfg->meshSizeI, (int)cfg->meshSizeK) == TOPO_SURFACE) {
+            if (cfg->meshSizeI > TOPO_MAX_I)
+                return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than %d nodes along x", TOPO_MAX_I);
+            if (cfg->meshSizeI > 1024 && cfg->meshSizeK > 1024)
+                return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 with more than 1024 nodes along both x and z");
+        }
         if (cfg->precision != SNH_PRECISION_FP64)
             return set_err(SNH_EUNSUPPORTED, "topo_kappa > 0 runs on the FP64 path");
     }
@@ -2804,9 +2808,14 @@ extern "C" int snh_set_element_material(snh_context* c, const uint8_t* mat, size
     HIPCHK(hipStreamSynchronize(c->stream));
     bool used[SNH_MAX_MATERIALS] = {};
     for (size_t e = 0; e < nelem; e++) used[mat[e]] = true;
-    c->c_wave = 0.0;
+    /* the reference's speed of sound is a running maximum over the elements
+       UpdateElementMomentum has seen (Context.c:392, UpdateElement.c:62-67):
+       before TimeStepZero it starts from the materials now in use, after it
+       only grows */
+    double cw = 0.0;
     for (size_t q = 0; q < c->mats.size(); q++)
-        if (used[q]) c->c_wave = wave_speed_max(c->c_wave, c->mats[q]);
+        if (used[q]) cw = wave_speed_max(cw, c->mats[q]);
+    c->c_wave = c->zero_done && c->c_wave > cw ? c->c_wave : cw;
     return SNH_OK;
 }
 

@@ -243,6 +243,25 @@ def test_autograd_function_gives_adjoint_gradient():
     J.backward()
     _, g, _, _, _ = gpu_gradient(p, param, n, interval, target)
     assert np.array_equal(theta.grad.numpy(), g)
+    # the record is released by the first backward: a second one says so
+    import pytest as _pt
+    with _pt.raises(RuntimeError, match="released after the first backward"):
+        J2 = 0.5 * ((up - torch.as_tensor(target)) ** 2).sum()
+        J2.backward()
+
+
+def test_autograd_function_without_gradient_runs_plain_steps():
+    """theta without requires_grad: plain steps, no adjoint record, the same
+    uplift as with the record."""
+    import torch
+    from snac_amd.autograd import surface_uplift
+    make, param, n, interval = CASES["maxwell"]
+    p = make()
+    theta0 = np.log(p.fields[_abi.FIELD_REFVISC])
+    a = surface_uplift(torch.tensor(theta0, dtype=torch.float64), p, param, n, interval)
+    b = surface_uplift(torch.tensor(theta0, dtype=torch.float64, requires_grad=True), p, param, n, interval)
+    assert not a.requires_grad
+    assert np.array_equal(a.numpy(), b.detach().numpy())
 
 
 def test_steps_after_a_recorded_forward_match_plain_steps():

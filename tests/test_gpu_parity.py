@@ -315,3 +315,26 @@ def test_node_pass_variants_match_the_oracle(case):
         gsc = c.scalars()
         c.Destroy()
         _check(f"{case}[node {variant}]", ref, got, rsc, gsc, fields=fields)
+
+
+def test_wave_speed_of_sound_is_a_running_maximum():
+    """After TimeStepZero the wave type's speed of sound only grows, as the
+    reference's running maximum (Context.c:392, UpdateElement.c:62-67):
+    moving every element to the slower material keeps it."""
+    from snac_amd import Context, _abi
+    p = P.mohr_coulomb_shear(elements=(6, 4, 6))
+    p.dtType = _abi.DT_WAVE
+    fast = P.MaterialSpec(rheology=_abi.RHEOLOGY_ELASTIC, alpha=0.0, beta=0.0, lam=6.0e10, mu=6.0e10,
+                          density=2500.0)
+    p.materials = [p.materials[0], fast]
+    mat = np.zeros(p.n_elements, dtype=np.uint8)
+    mat[: p.n_elements // 2] = 1
+    p.element_material = mat
+    c = Context(p).Construct().Build().Initialise()
+    c.Execute(3)
+    before = c.scalars()["speedOfSound"]
+    c.set_element_material(np.zeros(p.n_elements, dtype=np.uint8))
+    c.Execute(2)
+    after = c.scalars()["speedOfSound"]
+    c.Destroy()
+    assert before == after == pytest.approx((1.8e11 / 2500.0) ** 0.5, rel=1e-6)

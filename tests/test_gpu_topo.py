@@ -89,9 +89,9 @@ def test_parabola_at_rest_matches_the_oracle():
 
 @pytest.mark.parametrize("elements,bcs", [((1, 3, 12), False), ((12, 3, 1), False),
                                           ((1, 3, 12), True), ((12, 3, 1), True),
-                                          ((1, 3, 1), False), ((3000, 1, 1), False)],
+                                          ((1, 3, 1), False), ((3000, 1, 1), False), ((5000, 1, 1), False)],
                          ids=["along_z", "along_x", "along_z_compressed", "along_x_compressed",
-                              "two_by_two", "along_x_3001"])
+                              "two_by_two", "along_x_3001", "along_x_5001"])
 def test_quasi_two_dimensional_meshes(elements, bcs):
     """Two nodes wide in x (resp. z): the reference's diffTopo_alongZ
     (resp. _alongX) branch; two nodes wide in both: no diffusion."""
@@ -142,6 +142,8 @@ def _refused(make, text):
 def test_refused_where_not_built():
     base = lambda el: _with_topo(P.elastic_box_compression(elements=el))
     _refused(lambda: Context(base((4100, 1, 2))).Construct(), "nodes along x")
+    # the limit is the surface sweep's: two nodes wide in z the line mode runs
+    Context(base((4100, 1, 1))).Construct().Destroy()
     mixed = base((6, 3, 6))
     mixed.precision = _abi.PRECISION_MIXED
     _refused(lambda: Context(mixed).Construct(), "FP64 path")
