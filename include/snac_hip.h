@@ -358,7 +358,7 @@ int snh_set_diagnostics(snh_context* ctx, int enable);
 /* values of the last step run with diagnostics enabled (host arrays) */
 int snh_get_diagnostics(snh_context* ctx, const snh_diagnostics* out);
 
-/* Prescribe, for the next nsteps calls of the step (snh_step; FP64 path),
+/* Prescribe, for the next nsteps calls of the step (snh_step; FP64 and mixed),
  * dt and speed of sound of each step and optionally the damping branch
  * sign(v) sign(f) of every node component (damp: [nsteps][N][3] in
  * {-1, 0, 1}, or NULL): sign(v) in the damping term is replaced by
@@ -368,6 +368,16 @@ int snh_get_diagnostics(snh_context* ctx, const snh_diagnostics* out);
  * counterpart in the reference. */
 int snh_set_schedule(snh_context* ctx, int32_t nsteps, const double* dt, const double* speed_of_sound,
                      const int8_t* damp);
+
+/* Run nsteps steps (as snh_step) and record, per step, what
+ * snh_set_schedule prescribes: dt[nsteps] and speed_of_sound[nsteps] the
+ * step ran with and, when damp is not NULL, the damping branch
+ * sign(v) sign(f) of every node component (damp: [nsteps][N][3]).  Feeding
+ * the record to snh_set_schedule of another context with the same inputs
+ * replays the run bit for bit; a context of the other precision then
+ * follows the recorded run's discrete decisions.  Synchronous.  No
+ * counterpart in the reference. */
+int snh_record_schedule(snh_context* ctx, int32_t nsteps, double* dt, double* speed_of_sound, int8_t* damp);
 
 /* ---- adjoint of the explicit step ----------------------------------------
  * No counterpart in the reference (SNAC has no adjoint).  Reverse-mode

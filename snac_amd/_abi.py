@@ -211,6 +211,8 @@ SIGNATURES = {
     "snh_get_diagnostics": (_I, [_P, C.POINTER(Diagnostics)]),
     "snh_set_schedule": (_I, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_int8)]),
+    "snh_record_schedule": (_I, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int8)]),
     "snh_ensemble_create": (_I, [C.POINTER(_P), C.c_int32, C.c_int32, C.POINTER(_P)]),
     "snh_ensemble_step": (_I, [_P, C.c_int32]),
     "snh_ensemble_synchronize": (_I, [_P]),

@@ -175,6 +175,16 @@ class Context:
         self._check(self.lib.snh_set_schedule(self.h, n, _dp(dt), _dp(cs), dp))
         return self
 
+    def record_schedule(self, nsteps: int, damping: bool = True):
+        """Run nsteps steps recording what set_schedule prescribes: (dt [n],
+        speed of sound [n], damping branch [n, N, 3] int8 or None)."""
+        dt = np.zeros(nsteps)
+        cs = np.zeros(nsteps)
+        damp = np.zeros((nsteps, self.N, 3), dtype=np.int8) if damping else None
+        dp = damp.ctypes.data_as(C.POINTER(C.c_int8)) if damping else None
+        self._check(self.lib.snh_record_schedule(self.h, int(nsteps), _dp(dt), _dp(cs), dp))
+        return dt, cs, damp
+
     # -- adjoint ---------------------------------------------------------------
     def adjoint_begin(self, param: int, checkpoint_interval: int = 50, flags: int = 0):
         """Start recording for a reverse sweep with respect to a per-element

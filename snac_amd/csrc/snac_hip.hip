@@ -1252,12 +1252,15 @@ __device__ __forceinline__ double node_finish(const StepArgs& a, int n, double f
     const size_t N = (size_t)a.N;
     double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
     if (a.tape_f) {
-        /* what the reverse pass needs: the undamped force, the mass and
-           the damping branch of each component */
+        /* what the reverse pass needs: the undamped force and the mass */
         a.tape_f[n] = f0;
         a.tape_f[N + n] = f1;
         a.tape_f[2 * N + n] = f2;
         a.tape_m[n] = im;
+    }
+    if (a.tape_code) {
+        /* the damping branch of each component (the adjoint's tape, or a
+           recorded schedule) */
         const int c0 = (int)(GETSIGN(v0) * GETSIGN(f0)) + 1;
         const int c1 = (int)(GETSIGN(v1) * GETSIGN(f1)) + 1;
         const int c2 = (int)(GETSIGN(v2) * GETSIGN(f2)) + 1;
@@ -2367,6 +2370,8 @@ struct snh_context {
     uint8_t* sched_damp;      /* [sched_n][N] or null */
     uint32_t sched_base;
     int32_t sched_n;
+    uint8_t* rec_code;        /* [N] damping branches of a recorded step (snh_record_schedule) */
+    bool recording;           /* the node pass writes rec_code */
     /* kernel timing */
     bool profiling;
     std::vector<hipEvent_t> ev_pool;
@@ -2525,7 +2530,7 @@ static void free_all(snh_context* c) {
                     c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
                     c->dmat, c->dbc, c->diag_buf, c->force_buf, c->temp, c->temp0, c->tpre, c->hflux,
                     c->tstress, c->dens, c->tbc, c->tbcv, c->stage, c->topo_rec, c->topo_pmax,
-                    c->topo_pfirst};
+                    c->topo_pfirst, c->rec_code};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -3056,6 +3061,7 @@ static int launch_node(snh_context* c, int refStep) {
     StepArgs a = make_args(c);
     const int si = sched_index(c, c->timeStep);
     if (si >= 0 && c->sched_damp) a.damp_sched = c->sched_damp + (size_t)si * c->N;
+    if (c->recording) a.tape_code = c->rec_code;
     if (c->cfg.precision == SNH_PRECISION_MIXED) {
         hipLaunchKernelGGL(k_node_mx, dim3(c->n_node_blocks), dim3(NODE_BLOCK), 0, c->stream, a);
         c->n_vmax_parts = c->n_node_blocks;
@@ -3403,8 +3409,6 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
                                 const int8_t* damp) {
     if (!c) return set_err(SNH_EINVAL, "null context");
     if (nsteps < 0 || (nsteps > 0 && (!dt || !cs))) return set_err(SNH_EINVAL, "bad schedule");
-    if (nsteps > 0 && c->cfg.precision != SNH_PRECISION_FP64)
-        return set_err(SNH_EUNSUPPORTED, "schedules apply to the FP64 path");
     HIPCHK(hipStreamSynchronize(c->stream));
     void* old[] = {c->sched_dt, c->sched_c, c->sched_damp};
     for (void* p : old)
@@ -3441,6 +3445,55 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
    back to zero: done once a raised flag has been reported, and by every
    state upload, so that a caller who restarts from a checkpoint runs on.
    The context's stream is idle at both call sites. */
+extern "C" int snh_record_schedule(snh_context* c, int32_t nsteps, double* dt, double* cs, int8_t* damp) {
+    Range range_("snh_record_schedule");
+    if (!c) return set_err(SNH_EINVAL, "null context");
+    if (nsteps < 0 || (nsteps > 0 && (!dt || !cs))) return set_err(SNH_EINVAL, "bad arguments");
+    if (!c->zero_done) return set_err(SNH_ESTATE, "snh_record_schedule before snh_timestep_zero");
+    if (c->adj) return set_err(SNH_ESTATE, "snh_record_schedule while an adjoint record is open");
+    const size_t N = c->N;
+    std::vector<uint8_t> codes(damp ? N : 0);
+    if (damp && !c->rec_code) HIPCHK(hipMalloc((void**)&c->rec_code, N));
+    uint8_t* rec = c->rec_code;
+    int rc = SNH_OK;
+    for (int32_t s = 0; s < nsteps && rc == SNH_OK; s++) {
+        DevScalars sc;
+        /* dt and speed of sound the step runs with (a schedule, if one is
+           set, puts its own in place first, as snh_step does) */
+        const int si = sched_index(c, c->timeStep);
+        if (si >= 0) {
+            hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c->stream, c->scal, c->sched_dt + si,
+                               c->sched_c + si);
+            if (hipGetLastError() != hipSuccess) {
+                rc = set_err(SNH_EHIP, "schedule launch failed");
+                break;
+            }
+        }
+        if (hipMemcpyAsync(&sc, c->scal, sizeof sc, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = set_err(SNH_EHIP, "scalar download failed");
+            break;
+        }
+        dt[s] = sc.dt;
+        cs[s] = sc.speedOfSound;
+        c->recording = damp != nullptr;
+        rc = snh_step(c, 1);
+        c->recording = false;
+        if (rc) break;
+        if (damp) {
+            if (hipMemcpyAsync(codes.data(), rec, N, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = set_err(SNH_EHIP, "damping record download failed");
+                break;
+            }
+            int8_t* out = damp + (size_t)s * 3 * N;
+            for (size_t n = 0; n < N; n++)
+                for (int d = 0; d < 3; d++) out[3 * n + d] = (int8_t)((int)((codes[n] >> (2 * d)) & 3u) - 1);
+        }
+    }
+    return rc;   /* rec_code is kept for the next recording and freed with the context */
+}
+
 static int clear_firewalls(snh_context* c) {
     const int32_t zero[2] = {0, 0};
     HIPCHK(hipMemcpyAsync((char*)c->scal + offsetof(DevScalars, thermal_err), zero, sizeof zero,

@@ -365,28 +365,9 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
             if (a.dtType == SNH_DT_DYNAMIC) im += mi;
         }
         if (a.dtType != SNH_DT_DYNAMIC) im = 1.0; /* host rejects other dt types in mixed mode */
-        double v0 = a.v[n], v1 = a.v[NN + n], v2 = a.v[2 * NN + n];
-        f0 -= a.demf * GETSIGN(v0) * fabs(f0);
-        f1 -= a.demf * GETSIGN(v1) * fabs(f1);
-        f2 -= a.demf * GETSIGN(v2) * fabs(f2);
-        v0 += dt * f0 / im;
-        v1 += dt * f1 / im;
-        v2 += dt * f2 / im;
-        const uint32_t code = a.bc[n];
-        if (code) {
-            const DevBC& b = a.bcs[code];
-            if (b.mask & 1u) v0 = b.val[0];
-            if (b.mask & 2u) v1 = b.val[1];
-            if (b.mask & 4u) v2 = b.val[2];
-            vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-        }
-        if (!(isfinite(v0) && isfinite(v1) && isfinite(v2))) a.scal->nonfinite = 1;
-        a.v_out[n] = v0;
-        a.v_out[NN + n] = v1;
-        a.v_out[2 * NN + n] = v2;
-        a.xn[n] = xn0 + dt * v0;
-        a.xn[NN + n] = xn1 + dt * v1;
-        a.xn[2 * NN + n] = xn2 + dt * v2;
+        /* damping (or the prescribed damping branch), velocity, BC,
+           coordinates: the FP64 node update */
+        vmag = node_finish(a, n, f0, f1, f2, im, xn0, xn1, xn2, dt);
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;

@@ -5,8 +5,9 @@ The reference's local damping, f -= demf sign(v) |f| (UpdateNode.c:102-104),
 is discontinuous in v: with demf = 0.8 a 1e-12 relative perturbation of the
 FP64 initial stress already moves the surface by ~1e-3 of its displacement
 after a few hundred steps.  The arithmetic error of the mixed path is
-therefore measured undamped (demf = 0, bar: 1e-6 relative topography), and
-with damping it is compared with that FP64 self-sensitivity."""
+therefore measured undamped (demf = 0), and with damping along the FP64
+run's own recorded dt and damping branches; the bar is 1e-6 of the
+topography in both."""
 import numpy as np
 import pytest
 
@@ -72,17 +73,33 @@ def test_mixed_undamped_topography_error(make, variant):
     lambda: P.elastic_box_compression(elements=(16, 8, 16)),
     lambda: P.mohr_coulomb_extension(elements=(16, 6, 8), seed=1),
 ], ids=["maxwell", "elastic", "mohr_coulomb"])
-def test_mixed_damped_error_within_fp64_sensitivity(make):
+def test_mixed_damped_error_on_the_fp64_schedule(make):
+    """With the reference's damping, the mixed run follows the FP64 run's
+    recorded dt, speed of sound and damping branches (snh_record_schedule /
+    snh_set_schedule): its topography stays within 1e-6 of FP64's over 300
+    steps.  (Left to its own branches it would not: the damping sign(v) is
+    discontinuous, and a 1e-12 perturbation of FP64's own input already
+    moves the topography by ~1e-3 over those steps, printed for scale.)"""
     p = make()
-    x0, a, sa = _run(p, _abi.PRECISION_FP64, 300)
-    _, b, sb = _run(p, _abi.PRECISION_MIXED, 300)
+    p.precision = _abi.PRECISION_FP64
+    a = Context(p).Construct().Build().Initialise()
+    x0 = a.state(fields=("coord",))["coord"]
+    dt, cs, damp = a.record_schedule(300)
+    ra, sa = a.state(fields=("coord", "velocity", "stress", "plastic_strain")), a.scalars()
+    a.Destroy()
+    q = make()
+    q.precision = _abi.PRECISION_MIXED
+    b = Context(q).Construct().Build().Initialise()
+    b.set_schedule(dt, cs, damp)
+    b.Execute(300)
+    rb, sb = b.state(fields=("coord", "velocity", "stress", "plastic_strain")), b.scalars()
+    b.Destroy()
+    mixed = _errors(p, x0, ra, sa, rb, sb)
     _, c, sc = _run(p, _abi.PRECISION_FP64, 300, perturb=1e-12)
-    mixed = _errors(p, x0, a, sa, b, sb)
-    floor = _errors(p, x0, a, sa, c, sc)
-    _report(f"{p.name} demf=0.8 mixed vs fp64", mixed)
-    _report(f"{p.name} demf=0.8 fp64 with 1e-12 stress perturbation vs fp64", floor)
-    assert np.isfinite(mixed["topography"])
-    assert mixed["topography"] <= max(10.0 * floor["topography"], TOPO_TOL)
+    floor = _errors(p, x0, ra, sa, c, sc)
+    _report(f"{p.name} demf=0.8 mixed on the fp64 schedule vs fp64", mixed)
+    _report(f"{p.name} demf=0.8 fp64 with 1e-12 stress perturbation vs fp64 (for scale)", floor)
+    assert mixed["topography"] <= TOPO_TOL
 
 
 def test_mixed_layouts_agree():
@@ -104,3 +121,24 @@ def test_mixed_rejects_diagnostics():
             c.set_diagnostics(True)
     finally:
         c.Destroy()
+
+
+@pytest.mark.parametrize("prec", [_abi.PRECISION_FP64, _abi.PRECISION_MIXED], ids=["fp64", "mixed"])
+def test_recorded_schedule_replays_the_run_bit_for_bit(prec):
+    """snh_record_schedule's dt, speed of sound and damping branches fed to
+    snh_set_schedule of a fresh context with the same inputs give the same
+    run, bit for bit."""
+    p = P.mohr_coulomb_extension(elements=(12, 6, 8), seed=3)
+    p.precision = prec
+    a = Context(p).Construct().Build().Initialise()
+    dt, cs, damp = a.record_schedule(40)
+    sa = a.state()
+    a.Destroy()
+    assert np.all(dt > 0) and set(np.unique(damp)) <= {-1, 0, 1} and np.any(damp != 0)
+    b = Context(p).Construct().Build().Initialise()
+    b.set_schedule(dt, cs, damp)
+    b.Execute(40)
+    sb = b.state()
+    b.Destroy()
+    for f in ("coord", "velocity", "stress", "plastic_strain"):
+        assert np.array_equal(sa[f], sb[f]), f
