@@ -138,7 +138,8 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
         partial += sh_b[le][q];
     }
     const double elemVolume = evol;
-    const float srcorr = (float)(partial / elemVolume);
+    const double rvol = 1.0 / elemVolume;
+    const float srcorr = (float)(partial * rvol);
 
     double st[6];
 #pragma unroll
@@ -244,22 +245,23 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
         }
         const double v_new = tet_volume(Xc[0], Xc[1], Xc[2], Xc[3]);
         const double v_old = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
-        const double trace_strain = (v_new - v_old) / v_old;
+        /* a difference of nearly equal FP64 volumes, then an FP32 quotient */
+        const double trace_strain = (double)((float)(v_new - v_old) / (float)v_old);
         const double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
         double visc;
         {
-            const double te3 = trace_strain / 3.0;
+            const double te3 = trace_strain * (1.0 / 3.0);
             double d0 = sn[0] - te3, d1 = sn[1] - te3, d2 = sn[2] - te3;
             const double snd[6] = {sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]};
             visc = viscosity_of(m, rvisc, d0, d1, d2, snd, dt);
         }
         if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
         /* sigma'_new = sigma' + sigma'(-2x/(1+x)) + 2 mu e'/(1+x), x = mu dt/(2 eta) */
-        const double mean = (st[0] + st[1] + st[2]) / 3.0;
-        const float x = (float)(m.mu / (2.0 * visc) * dt);
+        const double mean = (st[0] + st[1] + st[2]) * (1.0 / 3.0);
+        const float x = (float)m.mu / (2.0f * (float)visc) * dtf;
         const float fdec = -2.0f * x / (1.0f + x);
         const float gmu = (float)(2.0 * m.mu) / (1.0f + x);
-        const float te3f = (float)(trace_strain / 3.0);
+        const float te3f = (float)trace_strain * (1.0f / 3.0f);
         const double bulk_inc = (m.lambda + 2.0 * m.mu / 3.0) * trace_strain;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -280,14 +282,14 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
     }
 
     /* stress trace averaging in FP64 (Stress.c:87-137) */
-    const double tr = (st[0] + st[1] + st[2]) / 3.0;
+    const double tr = (st[0] + st[1] + st[2]) * (1.0 / 3.0);
     __syncthreads();
     if (in_wave) sh_b[le][t] = tr * (double)k.V;
     __syncthreads();
     double pst = 0.0;
 #pragma unroll
     for (int q = 0; q < NTET; q++) pst += sh_b[le][q];
-    const double avgTrace = pst / elemVolume;
+    const double avgTrace = pst * rvol;
     st[0] += -tr + avgTrace;
     st[1] += -tr + avgTrace;
     st[2] += -tr + avgTrace;
@@ -298,14 +300,18 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
 }
 
 /* node gather, mixed precision */
-__global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
+__global__ void __launch_bounds__(NODE_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) k_node_mx(StepArgs a) {
     __shared__ double sh[NODE_BLOCK / 64];
-    const int n = xcd_block() * blockDim.x + threadIdx.x;
+    const int g = xcd_block() * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
-    if (n < a.N) {
+    if (g < a.N) {
+        /* the FP64 gather's banded node order: node rows that read one
+           element row run close together and share its L2 */
+        const int n = gather_node(a, g);
         const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
         const double dt = a.scal->dt;
         const double c = a.scal->speedOfSound;
+        const double rc2 = 1.0 / (c * c);
         const size_t E = (size_t)a.E;
         const size_t NN = (size_t)a.N;
         const double xn0 = a.xc[n], xn1 = a.xc[NN + n], xn2 = a.xc[2 * NN + n];
@@ -357,11 +363,12 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node_mx(StepArgs a) {
             }
             vs *= 1.0 / 6.0;
             const double gr = a.gravity * m.rho_eff * 0.125 * vs;
-            const double mi = 0.25 * (m.lam2mu / (c * c)) * vs;
+            const double mi = 0.25 * (m.lam2mu * rc2) * vs;
             /* sigma . n A / 6 = sigma . (2 n A) / 12 */
-            f0 += (sref[0] * C0 + sref[3] * C1 + sref[4] * C2) / 12.0 + (double)g0 / 12.0;
-            f1 += (sref[3] * C0 + sref[1] * C1 + sref[5] * C2) / 12.0 + (double)g1 / 12.0 - gr;
-            f2 += (sref[4] * C0 + sref[5] * C1 + sref[2] * C2) / 12.0 + (double)g2 / 12.0;
+            constexpr double r12 = 1.0 / 12.0;
+            f0 += (sref[0] * C0 + sref[3] * C1 + sref[4] * C2) * r12 + (double)g0 * r12;
+            f1 += (sref[3] * C0 + sref[1] * C1 + sref[5] * C2) * r12 + (double)g1 * r12 - gr;
+            f2 += (sref[4] * C0 + sref[5] * C1 + sref[2] * C2) * r12 + (double)g2 * r12;
             if (a.dtType == SNH_DT_DYNAMIC) im += mi;
         }
         if (a.dtType != SNH_DT_DYNAMIC) im = 1.0; /* host rejects other dt types in mixed mode */
