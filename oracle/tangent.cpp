@@ -18,9 +18,14 @@
  * (oracle/snac_oracle.c: orc_record_decisions): dt and speed of sound of each
  * step, sign(v) sign(f) of each damped node component and the plastic branch
  * of each tet.  Written independently of the GPU reverse kernels: geometry
- * from the face cross products a (n A = a / 2), eigenvalues of the trial
- * stress in closed form (trigonometric solution of the characteristic
- * cubic) and the return assembled with Sylvester's eigenprojectors.
+ * from the face cross products a (n A = a / 2); the return from the
+ * eigenpairs of the trial stress's values (the oracle's restatement of the
+ * reference's solver, whose quirks the trajectory depends on) with its
+ * derivative in eigenprojector form, written independently of the GPU
+ * reverse kernel's eigenbasis form.  A second form of the return, the
+ * closed-form eigenvalues (trigonometric solution of the characteristic
+ * cubic) with the dual-number Sylvester assembly, checks the first where the
+ * principal stresses are well apart.  Links liboracle.so.
  */
 #include <math.h>
 #include <stdint.h>
@@ -29,6 +34,8 @@
 #include <vector>
 
 #include "../include/snac_hip.h"
+
+#include "snac_oracle.h"
 
 namespace {
 
@@ -142,6 +149,134 @@ void eig3(const Dual S[3][3], Dual lam[3]) {
     lam[1] = 3.0 * q - lam[0] - lam[2];
 }
 
+/* Eigenpairs of the trial stress's values, ascending, with the projectors
+   P_k = v_k v_k^T, from the CPU oracle's restatement of the reference's
+   solver (Householder tridiagonalisation and QL iterations with the
+   reference's int abs(), plastic_BI/Constitutive.c:240-510).  Its int abs()
+   takes off-diagonals below 1 Pa as converged; where two principal stresses
+   lie within about 1 Pa of each other, the eigenvectors it returns, and with
+   them the return's result, depend on that, so the tangent model takes them
+   from the same solver to follow the reference's trajectory. */
+void reference_eigenpairs(const double A[3][3], double w[3], double Pk[3][3][3]) {
+    double V[3][3];
+    orc_principal_stresses(A, w, V);
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Pk[k][i][j] = V[i][k] * V[j][k];
+}
+
+/* Eigenpairs of a symmetric 3x3 (values) by cyclic Jacobi rotations,
+   ascending, with the projectors P_k = v_k v_k^T.  Rotations drive every
+   off-diagonal to zero (or below 1e-18 of its diagonal pair), so the
+   eigenpairs are smooth functions of the matrix wherever its eigenvalues are
+   distinct: the value map whose finite differences check the dual-number
+   derivative. */
+void jacobi3(const double A0[3][3], double w[3], double Pk[3][3][3]) {
+    double a[3][3], V[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            a[i][j] = A0[i][j];
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+    static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int sweep = 0; sweep < 64; sweep++) {
+        bool rotated = false;
+        for (const auto& pq : PQ) {
+            const int p = pq[0], q = pq[1];
+            if (a[p][q] == 0.0) continue;
+            if (fabs(a[p][q]) <= 1e-18 * (fabs(a[p][p]) + fabs(a[q][q]))) {
+                a[p][q] = a[q][p] = 0.0;
+                continue;
+            }
+            rotated = true;
+            /* t = tan of the angle that zeroes a_pq, the smaller root of
+               t^2 + 2 th t - 1 = 0, th = (a_qq - a_pp) / (2 a_pq) */
+            const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+            const double t = fabs(th) > 1e150 ? 0.5 / th
+                                              : (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+            for (int k = 0; k < 3; k++) {   /* A J */
+                const double akp = a[k][p], akq = a[k][q];
+                a[k][p] = c * akp - sn * akq;
+                a[k][q] = sn * akp + c * akq;
+            }
+            for (int k = 0; k < 3; k++) {   /* J^T (A J) */
+                const double apk = a[p][k], aqk = a[q][k];
+                a[p][k] = c * apk - sn * aqk;
+                a[q][k] = sn * apk + c * aqk;
+            }
+            a[p][q] = a[q][p] = 0.0;
+            for (int k = 0; k < 3; k++) {   /* V J */
+                const double vkp = V[k][p], vkq = V[k][q];
+                V[k][p] = c * vkp - sn * vkq;
+                V[k][q] = sn * vkp + c * vkq;
+            }
+        }
+        if (!rotated) break;
+    }
+    int order[3] = {0, 1, 2};
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++)
+            if (a[order[j]][order[j]] < a[order[i]][order[i]]) {
+                const int t = order[i];
+                order[i] = order[j];
+                order[j] = t;
+            }
+    for (int k = 0; k < 3; k++) {
+        const int o = order[k];
+        w[k] = a[o][o];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Pk[k][i][j] = V[i][o] * V[j][o];
+    }
+}
+
+/* The returned stress sum_k g_k P_k and its derivative in projector form:
+     d(sum_k g_k P_k) = sum_k dg_k P_k
+                        + sum_{k<l} G_kl (P_k dS P_l + P_l dS P_k),
+     G_kl = (g_k - g_l) / (lam_k - lam_l),
+   the second sum being sum_k g_k dP_k (dP_k = sum_{l != k} (P_k dS P_l +
+   P_l dS P_k) / (lam_k - lam_l)).  G_kl is the factor by which the return
+   scales the rotation of the eigenbasis; with clip it is held to [0, 1] (the
+   corner clamp of the adjoint, SNH_ADJ_CORNER_CLAMP), and it is 1 where the
+   two eigenvalues coincide to 1e-14 of their size. */
+void projector_return(const Dual S[3][3], const Dual lam[3], const Dual g[3], const double Pk[3][3][3], bool clip,
+                      Dual out[3][3]) {
+    double dS[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) dS[i][j] = S[i][j].d;
+    double v[3][3], d[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            v[i][j] = g[0].v * Pk[0][i][j] + g[1].v * Pk[1][i][j] + g[2].v * Pk[2][i][j];
+            d[i][j] = g[0].d * Pk[0][i][j] + g[1].d * Pk[1][i][j] + g[2].d * Pk[2][i][j];
+        }
+    for (int k = 0; k < 3; k++)
+        for (int l = k + 1; l < 3; l++) {
+            const double gap = lam[k].v - lam[l].v;
+            double G = 1.0;
+            if (fabs(gap) > 1e-14 * (fabs(lam[k].v) + fabs(lam[l].v))) G = (g[k].v - g[l].v) / gap;
+            if (clip) G = G < 0.0 ? 0.0 : (G > 1.0 ? 1.0 : G);
+            double A[3][3], B[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    A[i][j] = 0.0;
+                    B[i][j] = 0.0;
+                    for (int r = 0; r < 3; r++) {
+                        A[i][j] += Pk[k][i][r] * dS[r][j];
+                        B[i][j] += Pk[l][i][r] * dS[r][j];
+                    }
+                }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double t = 0.0;
+                    for (int r = 0; r < 3; r++) t += A[i][r] * Pk[l][r][j] + B[i][r] * Pk[k][r][j];
+                    d[i][j] += G * t;
+                }
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[i][j] = Dual(v[i][j], d[i][j]);
+}
+
 struct Mat {
     uint32_t rheology;
     double lambda, mu, rho, lam2mu;
@@ -170,7 +305,20 @@ typedef struct tg_problem {
     const double* bc_value;       /* [N][3] */
     double gravity, demf;
     int32_t param;                /* SNH_PARAM_* */
+    int32_t flags;                /* SNH_ADJ_CORNER_CLAMP, TG_SYLVESTER, TG_JACOBI */
+    void* follow;                 /* orc_context* at the start state, or NULL */
 } tg_problem;
+
+/* flags bit of tg_problem beside SNH_ADJ_CORNER_CLAMP: the Mohr-Coulomb
+   return through the closed-form eigenvalues (trigonometric solution of the
+   characteristic cubic) and the dual-number Sylvester assembly, in place of
+   the reference's eigenpairs and the projector form; a cross-check of the
+   two where the principal stresses are well apart.  Not combined with the
+   clamp. */
+#define TG_SYLVESTER 0x100
+/* flags bit: eigenpairs by Jacobi rotations (a smooth value map) in place of
+   the reference's solver */
+#define TG_JACOBI 0x200
 
 typedef struct tg_decisions {
     int32_t nsteps;
@@ -185,6 +333,7 @@ typedef struct tg_decisions {
    derivative along dir [N][3].  Returns 0, or -1 on a bad argument. */
 int tg_run(const tg_problem* P, const tg_decisions* D, const double* dir, double* x_out, double* dx_out) {
     if (!P || !D || !x_out || !dx_out || P->nmat < 1) return -1;
+    if ((P->flags & TG_SYLVESTER) && (P->flags & SNH_ADJ_CORNER_CLAMP)) return -1;
     const int I = P->I, J = P->J, K = P->K;
     const int EI = I - 1, EJ = J - 1, EK = K - 1;
     const size_t N = (size_t)I * J * K, E = (size_t)EI * EJ * EK;
@@ -235,8 +384,34 @@ int tg_run(const tg_problem* P, const tg_decisions* D, const double* dir, double
         return nid(ei + HexOff[q][0], ej + HexOff[q][1], ek + HexOff[q][2]);
     };
     std::vector<Dual> f(3 * N), mass(N), xn(3 * N);
+    std::vector<double> fx, fv, fs, fp;
+    orc_context* follow = (orc_context*)P->follow;
+    if (follow) {
+        if (orc_node_count(follow) != N || orc_element_count(follow) != E) return -1;
+        fx.resize(3 * N);
+        fv.resize(3 * N);
+        fs.resize(60 * E);
+        fp.resize(10 * E);
+    }
     for (int step = 0; step < D->nsteps; step++) {
         const double dt = D->dt[step], cs = D->c[step];
+        if (follow) {
+            /* linearise about the oracle's state of this step: the values
+               are the oracle's, the derivatives carry on */
+            snh_state os;
+            memset(&os, 0, sizeof os);
+            os.coord = fx.data();
+            os.velocity = fv.data();
+            os.stress = fs.data();
+            os.plastic_strain = fp.data();
+            if (orc_get_state(follow, &os) != 0) return -1;
+            for (size_t i = 0; i < 3 * N; i++) {
+                x[i].v = fx[i];
+                v[i].v = fv[i];
+            }
+            for (size_t i = 0; i < 60 * E; i++) st[i].v = fs[i];
+            for (size_t i = 0; i < 10 * E; i++) ps[i].v = fp[i];
+        }
         const int8_t* br = D->branch ? D->branch + (size_t)step * 10 * E : nullptr;
         const int8_t* dm = D->damp + (size_t)step * 3 * N;
         /* ---- strain rate, strain, constitutive, trace averaging ---- */
@@ -285,7 +460,24 @@ int tg_run(const tg_problem* P, const tg_decisions* D, const double* dir, double
                     if (b) {
                         Dual S[3][3] = {{s[0], s[3], s[4]}, {s[3], s[1], s[5]}, {s[4], s[5], s[2]}};
                         Dual lam[3];
-                        eig3(S, lam);
+                        double Pk[3][3][3];
+                        const bool sylvester = (P->flags & TG_SYLVESTER) != 0;
+                        if (sylvester) {
+                            eig3(S, lam);
+                        } else {
+                            double Sv[3][3], w[3];
+                            for (int i = 0; i < 3; i++)
+                                for (int j = 0; j < 3; j++) Sv[i][j] = S[i][j].v;
+                            if (P->flags & TG_JACOBI) jacobi3(Sv, w, Pk);
+                            else reference_eigenpairs(Sv, w, Pk);
+                            /* d lam_k = P_k : dS */
+                            for (int k = 0; k < 3; k++) {
+                                double dl = 0.0;
+                                for (int i = 0; i < 3; i++)
+                                    for (int j = 0; j < 3; j++) dl += Pk[k][i][j] * S[i][j].d;
+                                lam[k] = Dual(w[k], dl);
+                            }
+                        }
                         const double sphi = sin(m.phi * degrad), spsi = sin(m.psi * degrad);
                         const double anphi = (1.0 + sphi) / (1.0 - sphi), anpsi = (1.0 + spsi) / (1.0 - spsi);
                         const double a1 = m.lambda + 2.0 * m.mu, a2 = m.lambda;
@@ -313,25 +505,29 @@ int tg_run(const tg_problem* P, const tg_decisions* D, const double* dir, double
                         const double Kf = sqrt(0.5 * ((dep[0] - dm3) * (dep[0] - dm3) + (dep[1] - dm3) * (dep[1] - dm3) +
                                                       (dep[2] - dm3) * (dep[2] - dm3) + dm3 * dm3));
                         pst_t = pst_t + Kf * dabs(alam);
-                        /* sum_k g_k P_k, P_k = prod_{l != k} (S - lam_l) / (lam_k - lam_l) */
                         Dual out[3][3];
-                        for (int i = 0; i < 3; i++)
-                            for (int j = 0; j < 3; j++) out[i][j] = 0.0;
-                        for (int k = 0; k < 3; k++) {
-                            const int l1 = (k + 1) % 3, l2 = (k + 2) % 3;
-                            Dual A1[3][3], A2[3][3];
+                        if (sylvester) {
+                            /* sum_k g_k P_k, P_k = prod_{l != k} (S - lam_l) / (lam_k - lam_l) */
                             for (int i = 0; i < 3; i++)
-                                for (int j = 0; j < 3; j++) {
-                                    A1[i][j] = S[i][j] - (i == j ? lam[l1] : Dual(0.0));
-                                    A2[i][j] = S[i][j] - (i == j ? lam[l2] : Dual(0.0));
-                                }
-                            const Dual den = (lam[k] - lam[l1]) * (lam[k] - lam[l2]);
-                            for (int i = 0; i < 3; i++)
-                                for (int j = 0; j < 3; j++) {
-                                    Dual pr = 0.0;
-                                    for (int r = 0; r < 3; r++) pr += A1[i][r] * A2[r][j];
-                                    out[i][j] += g[k] * pr / den;
-                                }
+                                for (int j = 0; j < 3; j++) out[i][j] = 0.0;
+                            for (int k = 0; k < 3; k++) {
+                                const int l1 = (k + 1) % 3, l2 = (k + 2) % 3;
+                                Dual A1[3][3], A2[3][3];
+                                for (int i = 0; i < 3; i++)
+                                    for (int j = 0; j < 3; j++) {
+                                        A1[i][j] = S[i][j] - (i == j ? lam[l1] : Dual(0.0));
+                                        A2[i][j] = S[i][j] - (i == j ? lam[l2] : Dual(0.0));
+                                    }
+                                const Dual den = (lam[k] - lam[l1]) * (lam[k] - lam[l2]);
+                                for (int i = 0; i < 3; i++)
+                                    for (int j = 0; j < 3; j++) {
+                                        Dual pr = 0.0;
+                                        for (int r = 0; r < 3; r++) pr += A1[i][r] * A2[r][j];
+                                        out[i][j] += g[k] * pr / den;
+                                    }
+                            }
+                        } else {
+                            projector_return(S, lam, g, Pk, (P->flags & SNH_ADJ_CORNER_CLAMP) != 0, out);
                         }
                         s[0] = out[0][0];
                         s[1] = out[1][1];
@@ -427,6 +623,7 @@ int tg_run(const tg_problem* P, const tg_decisions* D, const double* dir, double
                 tet_rates(U, a, V, sr, w);
                 rotate(&st[(e * 10 + t) * 6], w, dt);
             }
+        if (follow && orc_step(follow, 1) != 0) return -1;
     }
     for (size_t i = 0; i < 3 * N; i++) {
         x_out[i] = x[i].v;

@@ -29,7 +29,12 @@ class TgProblem(C.Structure):
                 ("mat", C.c_void_p), ("materials", C.c_void_p), ("nmat", C.c_int32),
                 ("refvisc", C.c_void_p), ("cohoff", C.c_void_p),
                 ("bc_mask", C.c_void_p), ("bc_value", C.c_void_p),
-                ("gravity", C.c_double), ("demf", C.c_double), ("param", C.c_int32)]
+                ("gravity", C.c_double), ("demf", C.c_double), ("param", C.c_int32), ("flags", C.c_int32),
+                ("follow", C.c_void_p)]
+
+
+SYLVESTER = 0x100   # TG_SYLVESTER of tangent.cpp
+JACOBI = 0x200      # TG_JACOBI
 
 
 class TgDecisions(C.Structure):
@@ -81,8 +86,24 @@ def record(problem, nsteps, threads=4):
             "branch": branch, "final": final, "nsteps": nsteps}
 
 
-def tangent(problem, rec, param, direction=None, nsteps=None):
-    """(final coordinates [N, 3], derivative along `direction` [N, 3])."""
+def tangent(problem, rec, param, direction=None, nsteps=None, flags=0, follow=False, threads=4):
+    """(final coordinates [N, 3], derivative along `direction` [N, 3]).
+
+    flags: _abi.ADJ_CORNER_CLAMP differentiates the Mohr-Coulomb return with
+    the adjoint's corner clamp (the factor by which the return scales the
+    rotation of the trial stress's eigenbasis held to [0, 1]); JACOBI takes
+    the trial stress's eigenpairs from Jacobi rotations (a smooth value map,
+    for finite differences) instead of the reference's solver; SYLVESTER
+    takes the return through closed-form eigenvalues and the dual-number
+    Sylvester assembly (neither with the clamp).
+
+    follow: linearise about the CPU oracle's own trajectory, run beside the
+    model (`threads` threads): each step starts from the oracle's values, so
+    rounding differences between the two restatements do not build up where
+    the reference's return is sensitive to them (principal stresses within
+    ~1 Pa of each other, where its solver's int abs() decides the
+    eigenvectors).  The final coordinates returned are then the model's
+    last step from the oracle's state."""
     n = rec["nsteps"] if nsteps is None else nsteps
     I, J, K = problem.nodes
     cfg, keep = problem.to_c()
@@ -100,16 +121,25 @@ def tangent(problem, rec, param, direction=None, nsteps=None):
     val = np.ascontiguousarray(rec["bc_value"])
     P = TgProblem(I, J, K, _ptr(x0), _ptr(v0), _ptr(s0), _ptr(ps0), _ptr(mat),
                   C.cast(cfg.materials, C.c_void_p).value, cfg.nmaterials, _ptr(rv), _ptr(co),
-                  _ptr(mask), _ptr(val), cfg.gravity, cfg.demf, int(param))
+                  _ptr(mask), _ptr(val), cfg.gravity, cfg.demf, int(param), int(flags))
     dts = np.ascontiguousarray(rec["dt"])
     cs = np.ascontiguousarray(rec["c"])
     damp = np.ascontiguousarray(rec["damp"])
     branch = np.ascontiguousarray(rec["branch"])
     D = TgDecisions(n, _ptr(dts), _ptr(cs), _ptr(damp), _ptr(branch))
+    o = None
+    if follow:
+        o = Oracle(problem, threads=threads)
+        if problem.hydrostatic:
+            o.hydrostatic_ic()
+        o.timestep_zero()
+        P.follow = o.h
     d = None if direction is None else np.ascontiguousarray(direction, dtype=np.float64)
     x = np.zeros((problem.n_nodes, 3))
     dx = np.zeros((problem.n_nodes, 3))
     rc = lib().tg_run(C.byref(P), C.byref(D), _ptr(d), _ptr(x), _ptr(dx))
+    if o is not None:
+        o.close()
     if rc != 0:
         raise RuntimeError("tg_run failed")
     del keep

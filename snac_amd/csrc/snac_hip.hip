@@ -2380,6 +2380,22 @@ struct snh_context {
     size_t ev_used;
 };
 
+/* Host transfers and fills of a context's device buffers, ordered on the
+   context's stream: that stream is created non-blocking (or handed in by
+   the caller), so it does not wait for the null stream, and a plain
+   hipMemcpy / hipMemset there could overlap the stream's kernels.  A copy
+   runs after the stream's earlier work and is complete on return (the host
+   buffer may be reused, downloaded data may be read); a fill is ordered
+   before the stream's later work. */
+static hipError_t ctx_copy(snh_context* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+
+static hipError_t ctx_fill(snh_context* c, void* dst, int value, size_t bytes) {
+    return hipMemsetAsync(dst, value, bytes, c->stream);
+}
+
 enum { KK_ELEM = 0, KK_NODE = 1, KK_GEO = 2, KK_FIN = 3, KK_HEAT_ELEM = 4, KK_HEAT_NODE = 5, KK_KINDS = 6 };
 
 static hipEvent_t take_event(snh_context* c) {
@@ -2716,19 +2732,19 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
 
     int rc = SNH_OK;
     do {
-        if (hipMemcpy(c->dmat, dm.data(), sizeof(DevMaterial) * SNH_MAX_MATERIALS, hipMemcpyHostToDevice) !=
+        if (ctx_copy(c, c->dmat, dm.data(), sizeof(DevMaterial) * SNH_MAX_MATERIALS, hipMemcpyHostToDevice) !=
                 hipSuccess ||
-            hipMemcpy(c->dbc, table.data(), sizeof(DevBC) * table.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            ctx_copy(c, c->dbc, table.data(), sizeof(DevBC) * table.size(), hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_err(SNH_EHIP, "material / BC table upload failed");
             break;
         }
-        if (hipMemcpy(c->x[0], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->x[1], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->v, hv.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->bc, hbc.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(c->stress, 0, 60 * E * sizeof(double)) != hipSuccess ||
-            (c->ps && hipMemset(c->ps, 0, 10 * E * sizeof(double)) != hipSuccess) ||
-            (c->visc && hipMemset(c->visc, 0, 10 * E * sizeof(double)) != hipSuccess)) {
+        if (ctx_copy(c, c->x[0], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            ctx_copy(c, c->x[1], hx.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            ctx_copy(c, c->v, hv.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            ctx_copy(c, c->bc, hbc.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
+            ctx_fill(c, c->stress, 0, 60 * E * sizeof(double)) != hipSuccess ||
+            (c->ps && ctx_fill(c, c->ps, 0, 10 * E * sizeof(double)) != hipSuccess) ||
+            (c->visc && ctx_fill(c, c->visc, 0, 10 * E * sizeof(double)) != hipSuccess)) {
             rc = set_err(SNH_EHIP, "initial upload failed");
             break;
         }
@@ -2741,7 +2757,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
         s0.thermal_err = 0;
         s0.nonfinite = 0;
         s0.topoGradMax = 0.0;
-        if (hipMemcpy(c->scal, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
+        if (ctx_copy(c, c->scal, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_err(SNH_EHIP, "scalar upload failed");
             break;
         }
@@ -2907,11 +2923,11 @@ extern "C" int snh_upload_state(snh_context* c, const snh_state* s) {
         /* uploaded arrays are the state after LoopElementsMomentum (stress
            already rotated): cancel the pending rotation */
         const double zero = 0.0;
-        HIPCHK(hipMemcpy((char*)c->scal + offsetof(DevScalars, dt_rot), &zero, sizeof zero, hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(c, (char*)c->scal + offsetof(DevScalars, dt_rot), &zero, sizeof zero, hipMemcpyHostToDevice));
         c->rotation_pending = false;
     }
     if (s->temperature)
-        HIPCHK(hipMemcpy(c->temp, s->temperature, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(c, c->temp, s->temperature, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice));
     if (s->plastic_strain && (rc = upload_rows(c, s->plastic_strain, c->ps, E, XF_TET))) return rc;
     return clear_firewalls(c);
 }
@@ -3118,11 +3134,11 @@ static int launch_heat(snh_context* c, bool elem) {
    (Context.c:1009-1017) */
 static int fill_tet_density(snh_context* c) {
     std::vector<uint8_t> mat(c->E, 0);
-    if (c->mat) HIPCHK(hipMemcpy(mat.data(), c->mat, c->E, hipMemcpyDeviceToHost));
+    if (c->mat) HIPCHK(ctx_copy(c, mat.data(), c->mat, c->E, hipMemcpyDeviceToHost));
     std::vector<double> d(10 * (size_t)c->E);
     for (size_t e = 0; e < (size_t)c->E; e++)
         for (int t = 0; t < NTET; t++) d[(size_t)t * c->E + e] = c->cfg.materials[mat[e]].phsDensity;
-    HIPCHK(hipMemcpy(c->dens, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->dens, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
     return SNH_OK;
 }
 
@@ -3147,11 +3163,11 @@ extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
         if (!c->dens) HIPCHK(hipMalloc((void**)&c->dens, 10 * E * sizeof(double)));
         HIPCHK(hipMalloc((void**)&c->tbc, N));
         HIPCHK(hipMalloc((void**)&c->tbcv, 256 * sizeof(double)));
-        HIPCHK(hipMemset(c->hflux, 0, 30 * E * sizeof(double)));
+        HIPCHK(ctx_fill(c, c->hflux, 0, 30 * E * sizeof(double)));
     }
     if (th->computeThermalStress && !c->tstress) {
         HIPCHK(hipMalloc((void**)&c->tstress, 10 * E * sizeof(double)));
-        HIPCHK(hipMemset(c->tstress, 0, 10 * E * sizeof(double)));
+        HIPCHK(ctx_fill(c, c->tstress, 0, 10 * E * sizeof(double)));
     }
     /* _SnacTemperature_InitialConditions (VariableConditions.c:155-212):
        T0 = T, then the temperatureBCs (later entries win) */
@@ -3176,11 +3192,11 @@ extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
         code[n] = (uint8_t)k;
         T[n] = v;
     }
-    HIPCHK(hipMemcpy(c->temp, T.data(), N * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->temp0, T0.data(), N * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->tpre, T.data(), N * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->tbc, code.data(), N, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->tbcv, vals.data(), 256 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->temp, T.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->temp0, T0.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->tpre, T.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->tbc, code.data(), N, hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->tbcv, vals.data(), 256 * sizeof(double), hipMemcpyHostToDevice));
     c->thermal = true;
     c->thermal_stress = th->computeThermalStress != 0;
     return fill_tet_density(c);
@@ -3253,11 +3269,11 @@ extern "C" int snh_hydrostatic_ic(snh_context* c) {
     std::vector<double> x(3 * N), st(60 * E);
     std::vector<uint8_t> mat;
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(x.data(), c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(st.data(), c->stress, 60 * E * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(c, x.data(), c->x[c->cur], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(c, st.data(), c->stress, 60 * E * sizeof(double), hipMemcpyDeviceToHost));
     if (c->mat) {
         mat.resize(E);
-        HIPCHK(hipMemcpy(mat.data(), c->mat, E, hipMemcpyDeviceToHost));
+        HIPCHK(ctx_copy(c, mat.data(), c->mat, E, hipMemcpyDeviceToHost));
     }
     const int elx = c->EI, ely = c->EJ, elz = c->EK;
     auto yof = [&](int i, int j, int k) { return x[N + (size_t)(i + c->I * (j + c->J * k))]; };
@@ -3285,7 +3301,7 @@ extern "C" int snh_hydrostatic_ic(snh_context* c) {
                 rogh += dP;
             }
         }
-    HIPCHK(hipMemcpy(c->stress, st.data(), 60 * E * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->stress, st.data(), 60 * E * sizeof(double), hipMemcpyHostToDevice));
     c->rotation_pending = false;
     return SNH_OK;
 }
@@ -3381,8 +3397,8 @@ extern "C" int snh_set_diagnostics(snh_context* c, int enable) {
     if (enable && !c->diag_buf) {
         HIPCHK(hipMalloc((void**)&c->diag_buf, 5 * (size_t)c->E * sizeof(double)));
         HIPCHK(hipMalloc((void**)&c->force_buf, 3 * (size_t)c->N * sizeof(double)));
-        HIPCHK(hipMemset(c->diag_buf, 0, 5 * (size_t)c->E * sizeof(double)));
-        HIPCHK(hipMemset(c->force_buf, 0, 3 * (size_t)c->N * sizeof(double)));
+        HIPCHK(ctx_fill(c, c->diag_buf, 0, 5 * (size_t)c->E * sizeof(double)));
+        HIPCHK(ctx_fill(c, c->force_buf, 0, 3 * (size_t)c->N * sizeof(double)));
     }
     c->diag_on = enable != 0;
     return SNH_OK;
@@ -3395,10 +3411,10 @@ extern "C" int snh_get_diagnostics(snh_context* c, const snh_diagnostics* out) {
     HIPCHK(hipStreamSynchronize(c->stream));
     double* dst[4] = {out->strain_rate, out->stress, out->hydro_pressure, out->plastic_strain};
     for (int k = 0; k < 4; k++)
-        if (dst[k]) HIPCHK(hipMemcpy(dst[k], c->diag_buf + k * E, E * sizeof(double), hipMemcpyDeviceToHost));
+        if (dst[k]) HIPCHK(ctx_copy(c, dst[k], c->diag_buf + k * E, E * sizeof(double), hipMemcpyDeviceToHost));
     if (out->force) {
         std::vector<double> f(3 * N);
-        HIPCHK(hipMemcpy(f.data(), c->force_buf, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(ctx_copy(c, f.data(), c->force_buf, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
         for (size_t n = 0; n < N; n++)
             for (int d = 0; d < 3; d++) out->force[3 * n + d] = f[d * N + n];
     }
@@ -3420,8 +3436,8 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
     const size_t N = c->N;
     HIPCHK(hipMalloc((void**)&c->sched_dt, nsteps * sizeof(double)));
     HIPCHK(hipMalloc((void**)&c->sched_c, nsteps * sizeof(double)));
-    HIPCHK(hipMemcpy(c->sched_dt, dt, nsteps * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->sched_c, cs, nsteps * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->sched_dt, dt, nsteps * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(c, c->sched_c, cs, nsteps * sizeof(double), hipMemcpyHostToDevice));
     if (damp) {
         std::vector<uint8_t> packed((size_t)nsteps * N);
         for (size_t i = 0; i < packed.size(); i++) {
@@ -3434,7 +3450,7 @@ extern "C" int snh_set_schedule(snh_context* c, int32_t nsteps, const double* dt
             packed[i] = (uint8_t)code;
         }
         HIPCHK(hipMalloc((void**)&c->sched_damp, packed.size()));
-        HIPCHK(hipMemcpy(c->sched_damp, packed.data(), packed.size(), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(c, c->sched_damp, packed.data(), packed.size(), hipMemcpyHostToDevice));
     }
     c->sched_base = c->timeStep;
     c->sched_n = nsteps;
@@ -3508,7 +3524,7 @@ static int clear_firewalls(snh_context* c) {
    each raised flag is reported once */
 static int check_firewalls(snh_context* c) {
     int32_t flags[2] = {0, 0};
-    HIPCHK(hipMemcpy(flags, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof flags, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(c, flags, (char*)c->scal + offsetof(DevScalars, thermal_err), sizeof flags, hipMemcpyDeviceToHost));
     if (!flags[0] && !flags[1]) return SNH_OK;
     int rc;
     if ((rc = clear_firewalls(c))) return rc;
@@ -3549,7 +3565,7 @@ extern "C" int snh_download_state(snh_context* c, snh_state* s) {
     if (s->temperature && !c->thermal)
         return set_err(SNH_EINVAL, "no temperature plugin in this context (snh_set_thermal)");
     if (s->coord && (rc = download_rows(c, c->x[c->cur], s->coord, N, XF_VEC3))) return rc;
-    if (s->temperature) HIPCHK(hipMemcpy(s->temperature, c->temp, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (s->temperature) HIPCHK(ctx_copy(c, s->temperature, c->temp, N * sizeof(double), hipMemcpyDeviceToHost));
     if (s->velocity && (rc = download_rows(c, c->v, s->velocity, N, XF_VEC3))) return rc;
     if (s->stress || s->volume) {
         if (!c->tmp) HIPCHK(hipMalloc((void**)&c->tmp, 60 * E * sizeof(double)));
@@ -3586,12 +3602,12 @@ extern "C" int snh_download_state(snh_context* c, snh_state* s) {
             std::vector<double> rv;
             if (c->refvisc) {
                 rv.resize(E);
-                HIPCHK(hipMemcpy(rv.data(), c->refvisc, E * sizeof(double), hipMemcpyDeviceToHost));
+                HIPCHK(ctx_copy(c, rv.data(), c->refvisc, E * sizeof(double), hipMemcpyDeviceToHost));
             }
             std::vector<uint8_t> mat;
             if (c->mat) {
                 mat.resize(E);
-                HIPCHK(hipMemcpy(mat.data(), c->mat, E, hipMemcpyDeviceToHost));
+                HIPCHK(ctx_copy(c, mat.data(), c->mat, E, hipMemcpyDeviceToHost));
             }
             for (size_t e = 0; e < E; e++) {
                 const snh_material& m = c->mats[c->mat ? mat[e] : 0];

@@ -114,7 +114,7 @@ extern "C" int snh_adjoint_begin(snh_context* c, const snh_adjoint_config* cfg) 
     if (cfg->param == SNH_PARAM_COHESION && !c->cohoff) {
         /* a zero offset leaves every forward result unchanged */
         HIPCHK(hipMalloc((void**)&c->cohoff, E * sizeof(double)));
-        HIPCHK(hipMemset(c->cohoff, 0, E * sizeof(double)));
+        HIPCHK(ctx_fill(c, c->cohoff, 0, E * sizeof(double)));
     }
     AdjointCtx* A = new AdjointCtx();
     A->cfg = *cfg;

@@ -147,6 +147,30 @@ def test_corner_clamp():
         assert np.all(np.isfinite(gc))
 
 
+def test_clamped_adjoint_matches_clamped_tangent():
+    """300 steps of the yielding shear box, where the clamp bounds the
+    gradient: the adjoint with SNH_ADJ_CORNER_CLAMP against the tangent model
+    with the same clamp (written in projector form, independently of the
+    reverse kernel's eigenbasis form)."""
+    p = P.mohr_coulomb_shear(elements=(6, 4, 6))
+    target = _target(p)
+    n = 300
+    J, g, r, x, _ = gpu_gradient(p, _abi.PARAM_COHESION, n, 50, target, flags=_abi.ADJ_CORNER_CLAMP)
+    rec = T.record(p, n)
+    top = top_nodes(p)
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for k in range(3):
+        d = rng.standard_normal(p.n_elements)
+        _, dx = T.tangent(p, rec, _abi.PARAM_COHESION, d, flags=_abi.ADJ_CORNER_CLAMP)
+        dj_t = float(r @ dx[top, 1])
+        dj_a = float(g @ d)
+        rel = abs(dj_a - dj_t) / max(abs(dj_t), 1e-300)
+        worst = max(worst, rel)
+        print(f"    dir {k}: adjoint {dj_a:.12e} tangent {dj_t:.12e} rel {rel:.2e}", flush=True)
+    assert worst <= 1e-6
+
+
 def test_adjoint_forward_equals_plain_steps():
     make, param, n, interval = CASES["mc_shear"]
     p = make()
