@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SNH_ABI_VERSION 4
+#define SNH_ABI_VERSION 5
 
 /* ---- status codes ------------------------------------------------------- */
 #define SNH_OK            0
@@ -388,11 +388,19 @@ int snh_record_schedule(snh_context* ctx, int32_t nsteps, double* dt, double* sp
  * damping branch sign(v) sign(f) of every node component, the plastic branch
  * of every tet (elastic / shear / tension) and the viscosity clamp.
  *
- * Memory: checkpoints of the full state every checkpoint_interval steps, and
- * a tape of checkpoint_interval steps (stress, plastic strain, coordinates,
- * velocity, undamped force, mass) that each segment is replayed into before
- * its reverse sweep: two-level checkpointing, every forward step is run
- * twice in total.
+ * Schedule.  The steps fall into blocks of checkpoint_interval steps; a
+ * block is replayed from the state entering it into a tape (stress, plastic
+ * strain, coordinates, velocity, undamped force, mass of each step, about
+ * 750 B per element and step) and then reversed.  With snapshots == 0
+ * (uniform two-level checkpointing) snh_adjoint_forward keeps the full
+ * state entering every block (about 560 B per element each) and every step
+ * runs twice in total.  With snapshots > 0 it keeps only the state it
+ * starts from, and the reverse sweep follows the binomial checkpointing
+ * schedule (Griewank's revolve) over the blocks with at most `snapshots`
+ * further states at a time, replaying the fewest blocks that budget allows:
+ * memory is the tape of one block and snapshots + 1 states, whatever the
+ * number of steps.  Both schedules replay every step from the same state
+ * with the same kernels, so they give the same gradient bits.
  *
  * Supported: FP64, dynamic dt, elastic, plastic (plastic_BI semantics,
  * friction and dilation angles constant along the softening table) and
@@ -412,21 +420,26 @@ int snh_record_schedule(snh_context* ctx, int32_t nsteps, double* dt, double* sp
 
 typedef struct snh_adjoint_config {
     int32_t param;                /* SNH_PARAM_*                                  */
-    int32_t checkpoint_interval;  /* steps per checkpoint segment, >= 1           */
+    int32_t checkpoint_interval;  /* steps per block (tape length), >= 1          */
     int32_t flags;                /* SNH_ADJ_*                                    */
+    int32_t snapshots;            /* 0: a state kept per block; > 0: binomial     */
+                                  /* schedule with this many states beside the    */
+                                  /* first, allocated by snh_adjoint_begin        */
 } snh_adjoint_config;
 
 typedef struct snh_adjoint_times {
-    double forward_ms;   /* snh_adjoint_forward, checkpoint copies included */
-    double replay_ms;    /* forward replays of the reverse sweep            */
-    double reverse_ms;   /* reverse kernels                                 */
-    int32_t steps;       /* steps recorded                                  */
-    int32_t checkpoints;
+    double forward_ms;    /* snh_adjoint_forward, checkpoint copies included */
+    double replay_ms;     /* forward replays of the reverse sweeps           */
+    double reverse_ms;    /* reverse kernels                                 */
+    int32_t steps;        /* steps recorded                                  */
+    int32_t checkpoints;  /* states held: blocks (uniform) or snapshots + 1  */
+    int64_t replay_steps; /* steps replayed by the reverse sweeps            */
 } snh_adjoint_times;
 
 /* Start recording at the current state (after snh_timestep_zero). */
 int snh_adjoint_begin(snh_context* ctx, const snh_adjoint_config* cfg);
-/* Run nsteps explicit steps (same results as snh_step), storing checkpoints. */
+/* Run nsteps explicit steps (same results as snh_step), storing checkpoints;
+ * may be called several times before snh_adjoint_reverse, not after it. */
 int snh_adjoint_forward(snh_context* ctx, int32_t nsteps);
 /* Reverse sweep over all recorded steps.  coord_bar / vel_bar: d(misfit)/d
  * (final coordinates / velocities), [N][3] node-major like snh_state, either

@@ -167,12 +167,13 @@ ADJ_CORNER_CLAMP = 0x1
 
 
 class AdjointConfig(C.Structure):
-    _fields_ = [("param", C.c_int32), ("checkpoint_interval", C.c_int32), ("flags", C.c_int32)]
+    _fields_ = [("param", C.c_int32), ("checkpoint_interval", C.c_int32), ("flags", C.c_int32),
+                ("snapshots", C.c_int32)]
 
 
 class AdjointTimes(C.Structure):
     _fields_ = [("forward_ms", C.c_double), ("replay_ms", C.c_double), ("reverse_ms", C.c_double),
-                ("steps", C.c_int32), ("checkpoints", C.c_int32)]
+                ("steps", C.c_int32), ("checkpoints", C.c_int32), ("replay_steps", C.c_int64)]
 
 
 # Every function the header declares, with its ctypes signature.
@@ -225,7 +226,7 @@ SIGNATURES = {
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 
-ABI_VERSION = 4          # SNH_ABI_VERSION of include/snac_hip.h
+ABI_VERSION = 5          # SNH_ABI_VERSION of include/snac_hip.h
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libsnac_hip.so")

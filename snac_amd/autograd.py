@@ -51,7 +51,7 @@ def top_nodes(problem: Problem) -> np.ndarray:
 
 class SurfaceUplift(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, problem, param, nsteps, checkpoint_interval, flags):
+    def forward(ctx, theta, problem, param, nsteps, checkpoint_interval, flags, snapshots):
         p = with_parameter(problem, param, theta.detach().cpu().numpy())
         c = Context(p).Construct().Build().Initialise()
         top = top_nodes(p)
@@ -66,7 +66,7 @@ class SurfaceUplift(torch.autograd.Function):
             finally:
                 c.Destroy()
             return torch.as_tensor(y - y0, dtype=theta.dtype, device=theta.device)
-        c.adjoint_begin(param, checkpoint_interval, flags)
+        c.adjoint_begin(param, checkpoint_interval, flags, snapshots)
         c.adjoint_forward(nsteps)
         y = c.state(fields=("coord",))["coord"][top, 1]
         ctx.snac = c
@@ -86,9 +86,13 @@ class SurfaceUplift(torch.autograd.Function):
             c.adjoint_end()
             c.Destroy()
             ctx.snac = None
-        return torch.as_tensor(g, dtype=grad_uplift.dtype, device=grad_uplift.device), None, None, None, None, None
+        return (torch.as_tensor(g, dtype=grad_uplift.dtype, device=grad_uplift.device),
+                None, None, None, None, None, None)
 
 
 def surface_uplift(theta: torch.Tensor, problem: Problem, param: int, nsteps: int,
-                   checkpoint_interval: int = 50, flags: int = 0) -> torch.Tensor:
-    return SurfaceUplift.apply(theta, problem, param, nsteps, checkpoint_interval, flags)
+                   checkpoint_interval: int = 50, flags: int = 0, snapshots: int = 0) -> torch.Tensor:
+    """Uplift of the top-surface nodes after nsteps, differentiable in the
+    per-element parameter theta (the adjoint; checkpoint_interval, flags and
+    snapshots as for Context.adjoint_begin)."""
+    return SurfaceUplift.apply(theta, problem, param, nsteps, checkpoint_interval, flags, snapshots)

@@ -186,11 +186,13 @@ class Context:
         return dt, cs, damp
 
     # -- adjoint ---------------------------------------------------------------
-    def adjoint_begin(self, param: int, checkpoint_interval: int = 50, flags: int = 0):
+    def adjoint_begin(self, param: int, checkpoint_interval: int = 50, flags: int = 0, snapshots: int = 0):
         """Start recording for a reverse sweep with respect to a per-element
         field (_abi.PARAM_LOG_VISCOSITY or _abi.PARAM_COHESION); flags:
-        _abi.ADJ_CORNER_CLAMP."""
-        cfg = _abi.AdjointConfig(int(param), int(checkpoint_interval), int(flags))
+        _abi.ADJ_CORNER_CLAMP.  checkpoint_interval: steps per taped block;
+        snapshots: 0 keeps the state entering every block, n > 0 follows the
+        binomial (revolve) schedule with n states beside the first."""
+        cfg = _abi.AdjointConfig(int(param), int(checkpoint_interval), int(flags), int(snapshots))
         self._check(self.lib.snh_adjoint_begin(self.h, C.byref(cfg)))
         return self
 
@@ -218,7 +220,7 @@ class Context:
         t = _abi.AdjointTimes()
         self._check(self.lib.snh_adjoint_get_times(self.h, C.byref(t)))
         return {"forward_ms": t.forward_ms, "replay_ms": t.replay_ms, "reverse_ms": t.reverse_ms,
-                "steps": t.steps, "checkpoints": t.checkpoints}
+                "steps": t.steps, "checkpoints": t.checkpoints, "replay_steps": t.replay_steps}
 
     def adjoint_end(self):
         self._check(self.lib.snh_adjoint_end(self.h))

@@ -3,11 +3,28 @@
  * snh_adjoint_*): checkpoints, the replay tape and the reverse sweep.
  * Included at the end of snac_hip.hip.
  *
- * Schedule (two-level checkpointing): the recorded forward run stores the
- * state entering every checkpoint_interval-th step.  The reverse sweep walks
- * the segments backwards; each segment is replayed from its checkpoint into
+ * The steps fall into blocks of checkpoint_interval steps.  A block is
+ * reversed from a tape: its steps replayed from the state entering it into
  * tape slots (the forward kernels read slot i and write slot i+1, so the
- * tape costs no extra traffic) and then reversed step by step.
+ * tape costs no extra traffic), then reversed step by step.  Two schedules
+ * provide the state entering each block:
+ *
+ *   uniform (snapshots == 0): the recorded forward run stores it for every
+ *   block; the reverse sweep replays each block once.
+ *
+ *   binomial (snapshots > 0): the forward run stores only the state it
+ *   starts from, and the reverse sweep follows the binomial checkpointing
+ *   schedule (Griewank's revolve) over the blocks with at most `snapshots`
+ *   further states held at a time: to reverse blocks [a, b) from the state
+ *   entering a, it advances to a + m, keeps that state, reverses [a + m, b)
+ *   with one snapshot less, then [a, a + m) with the freed one.  m minimises
+ *   the blocks advanced, m + F(b - a - m, s - 1) + F(m, s), where F(n, s) =
+ *   t n - C(s + 1 + t, t - 1) is the minimum for n blocks and s snapshots
+ *   (t the least integer with C(s + 1 + t, t) >= n; with none, n(n - 1)/2).
+ *   Memory: the tape of one block and s + 1 states, however many steps.
+ *
+ * Either way every step is replayed by the same kernels from the same
+ * state, so both schedules give the same gradient bits.
  */
 #ifndef SNH_ADJOINT_HOST_H
 #define SNH_ADJOINT_HOST_H
@@ -25,7 +42,9 @@ struct AdjCheckpoint {
 struct AdjointCtx {
     snh_adjoint_config cfg;
     int32_t steps = 0;
-    std::vector<AdjCheckpoint> ckpt;
+    std::vector<AdjCheckpoint> ckpt;   /* uniform: one per block; binomial: the first */
+    std::vector<AdjCheckpoint> snap;   /* binomial: the snapshot pool               */
+    int64_t replay_steps = 0;          /* steps replayed by reverse sweeps          */
     /* tape of one segment */
     int K = 0;
     double* ts = nullptr;       /* (K+1) x [6][10][E] */
@@ -43,20 +62,37 @@ struct AdjointCtx {
     double forward_ms = 0.0, replay_ms = 0.0, reverse_ms = 0.0;
     int tape_seg = -1;          /* segment whose steps the tape holds      */
     int tape_len = 0;           /* steps of it recorded                    */
+    bool reversed = false;      /* a reverse sweep has used the tape       */
 };
+
+static void adj_free_state(AdjCheckpoint& k) {
+    void* p[] = {k.stress, k.ps, k.x, k.xp, k.v, k.scal};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    k = AdjCheckpoint();
+}
 
 static void adj_free(AdjointCtx* A) {
     if (!A) return;
-    for (AdjCheckpoint& k : A->ckpt) {
-        void* p[] = {k.stress, k.ps, k.x, k.xp, k.v, k.scal};
-        for (void* q : p)
-            if (q) (void)hipFree(q);
-    }
+    for (AdjCheckpoint& k : A->ckpt) adj_free_state(k);
+    for (AdjCheckpoint& k : A->snap) adj_free_state(k);
     void* p[] = {A->ts, A->tp, A->tx, A->tv, A->tf, A->tm, A->tc, A->hist, A->swork,
                  A->sbar, A->psbar, A->xbar, A->vbar, A->fbar, A->mbar, A->nb, A->grad};
     for (void* q : p)
         if (q) (void)hipFree(q);
     delete A;
+}
+
+/* device buffers of one state (stress, plastic strain, x, x_prev, v,
+   scalars); false when one cannot be allocated (what was is kept in k) */
+static bool adj_alloc_state(snh_context* c, AdjCheckpoint& k) {
+    const size_t E = c->E, N = c->N;
+    return hipMalloc((void**)&k.stress, 60 * E * sizeof(double)) == hipSuccess &&
+           (!c->ps || hipMalloc((void**)&k.ps, 10 * E * sizeof(double)) == hipSuccess) &&
+           hipMalloc((void**)&k.x, 3 * N * sizeof(double)) == hipSuccess &&
+           hipMalloc((void**)&k.xp, 3 * N * sizeof(double)) == hipSuccess &&
+           hipMalloc((void**)&k.v, 3 * N * sizeof(double)) == hipSuccess &&
+           hipMalloc((void**)&k.scal, sizeof(DevScalars)) == hipSuccess;
 }
 
 __global__ void k_copy_scalars(const DevScalars* __restrict__ src, DevScalars* __restrict__ dst) {
@@ -92,6 +128,7 @@ extern "C" int snh_adjoint_begin(snh_context* c, const snh_adjoint_config* cfg) 
     if (c->cfg.dtType == SNH_DT_WAVE)
         return set_err(SNH_EUNSUPPORTED, "the adjoint's mass reverse is the dynamic dt type's (dtType wave)");
     if (cfg->checkpoint_interval < 1) return set_err(SNH_EINVAL, "checkpoint_interval must be >= 1");
+    if (cfg->snapshots < 0) return set_err(SNH_EINVAL, "snapshots must be >= 0");
     if (cfg->param != SNH_PARAM_LOG_VISCOSITY && cfg->param != SNH_PARAM_COHESION)
         return set_err(SNH_EINVAL, "unknown adjoint parameter %d", cfg->param);
     if (c->cfg.precision != SNH_PRECISION_FP64) return set_err(SNH_EUNSUPPORTED, "adjoint: FP64 path only");
@@ -137,6 +174,13 @@ extern "C" int snh_adjoint_begin(snh_context* c, const snh_adjoint_config* cfg) 
     ADJ_ALLOC(A->mbar, N * sizeof(double));
     ADJ_ALLOC(A->nb, E * 8 * NBC * sizeof(double));
     ADJ_ALLOC(A->grad, E * sizeof(double));
+    for (int k = 0; k < cfg->snapshots; k++) {
+        A->snap.emplace_back();
+        if (!adj_alloc_state(c, A->snap.back())) {
+            adj_free(A);
+            return set_err(SNH_ENOMEM, "adjoint: cannot allocate snapshot %d of %d", k + 1, cfg->snapshots);
+        }
+    }
     c->adj = A;
     return SNH_OK;
 }
@@ -190,8 +234,9 @@ static int adj_copy_state(snh_context* c, double* ds, double* dp, double* dx, do
     const size_t E = c->E, N = c->N;
     HIPCHK(hipMemcpyAsync(ds, ss, 60 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if (dp && sp) HIPCHK(hipMemcpyAsync(dp, sp, 10 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(dx, sx, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    /* x_prev first: rolling tape slot 1 to slot 0 reads x_prev where x goes */
     HIPCHK(hipMemcpyAsync(dxp, sxp, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dx, sx, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(dv, sv, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_copy_scalars, dim3(1), dim3(1), 0, c->stream, ssc, dsc);
     HIPCHK(hipGetLastError());
@@ -203,17 +248,35 @@ static int adj_copy_state(snh_context* c, double* ds, double* dp, double* dx, do
 static int adj_save_checkpoint(snh_context* c, AdjointCtx* A, uint32_t t, const double* ss, const double* sp,
                                const double* sx, const double* sxp, const double* sv, const DevScalars* ssc) {
     AdjCheckpoint k;
-    const size_t E = c->E, N = c->N;
     k.timeStep = t;
-    bool ok = hipMalloc((void**)&k.stress, 60 * E * sizeof(double)) == hipSuccess &&
-              (!c->ps || hipMalloc((void**)&k.ps, 10 * E * sizeof(double)) == hipSuccess) &&
-              hipMalloc((void**)&k.x, 3 * N * sizeof(double)) == hipSuccess &&
-              hipMalloc((void**)&k.xp, 3 * N * sizeof(double)) == hipSuccess &&
-              hipMalloc((void**)&k.v, 3 * N * sizeof(double)) == hipSuccess &&
-              hipMalloc((void**)&k.scal, sizeof(DevScalars)) == hipSuccess;
+    const bool ok = adj_alloc_state(c, k);
     A->ckpt.push_back(k);   /* freed with the context's adjoint either way */
     if (!ok) return set_err(SNH_ENOMEM, "adjoint: cannot allocate checkpoint %zu", A->ckpt.size());
     return adj_copy_state(c, k.stress, k.ps, k.x, k.xp, k.v, k.scal, ss, sp, sx, sxp, sv, ssc);
+}
+
+/* state -> tape slot 0 */
+static int adj_load(snh_context* c, AdjointCtx* A, const AdjCheckpoint& k) {
+    const size_t N = c->N;
+    return adj_copy_state(c, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv, A->swork, k.stress, k.ps, k.x, k.xp, k.v,
+                          k.scal);
+}
+
+/* tape slot 0 -> state */
+static int adj_store(snh_context* c, AdjointCtx* A, AdjCheckpoint& k, uint32_t t) {
+    const size_t N = c->N;
+    k.timeStep = t;
+    return adj_copy_state(c, k.stress, k.ps, k.x, k.xp, k.v, k.scal, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv,
+                          A->swork);
+}
+
+/* tape slot L (the state after L taped steps) -> tape slot 0 */
+static int adj_roll(snh_context* c, AdjointCtx* A, int L) {
+    const size_t E = c->E, N = c->N;
+    if (L == 0) return SNH_OK;
+    return adj_copy_state(c, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv, A->swork, A->ts + (size_t)L * 60 * E,
+                          A->tp ? A->tp + (size_t)L * 10 * E : nullptr, A->tx + (size_t)(L + 1) * 3 * N,
+                          A->tx + (size_t)L * 3 * N, A->tv + (size_t)L * 3 * N, A->swork);
 }
 
 extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
@@ -222,6 +285,7 @@ extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
     AdjointCtx* A = c->adj;
     if (!A) return set_err(SNH_ESTATE, "snh_adjoint_forward before snh_adjoint_begin");
     if (nsteps < 0) return set_err(SNH_EINVAL, "negative step count");
+    if (A->reversed) return set_err(SNH_ESTATE, "snh_adjoint_forward after snh_adjoint_reverse");
     if (nsteps == 0) return SNH_OK;
     if (A->steps > 0 && c->timeStep != A->ckpt[0].timeStep + (uint32_t)A->steps)
         return set_err(SNH_ESTATE, "steps were run outside snh_adjoint_forward since snh_adjoint_begin");
@@ -237,7 +301,19 @@ extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
        tape_len afterwards */
     bool in_tape = false;
     while (done < nsteps && rc == SNH_OK) {
-        if (A->steps % A->K == 0) {
+        if (A->steps % A->K == 0 && A->steps > 0 && !A->snap.empty() && in_tape) {
+            /* binomial schedule: only the first state is kept */
+            if ((rc = adj_roll(c, A, A->tape_len))) break;
+            A->tape_seg = A->steps / A->K;
+            A->tape_len = 0;
+        } else if (A->steps % A->K == 0 && A->steps > 0 && !A->snap.empty()) {
+            /* a new call at a block boundary: the state is in the context */
+            rc = adj_copy_state(c, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv, A->swork, c->stress, c->ps,
+                                c->x[c->cur], c->x[c->cur ^ 1], c->v, c->scal);
+            if (rc) break;
+            A->tape_seg = A->steps / A->K;
+            A->tape_len = 0;
+        } else if (A->steps % A->K == 0) {
             const int L = A->tape_len;
             const double* ss = in_tape ? A->ts + (size_t)L * 60 * E : c->stress;
             const double* sp = in_tape ? (A->tp ? A->tp + (size_t)L * 10 * E : nullptr) : c->ps;
@@ -255,7 +331,7 @@ extern "C" int snh_adjoint_forward(snh_context* c, int32_t nsteps) {
         }
         in_tape = true;
         const int32_t todo = std::min(nsteps - done, A->K - A->steps % A->K);
-        const uint32_t t0 = A->ckpt[A->tape_seg].timeStep;
+        const uint32_t t0 = A->ckpt[0].timeStep + (uint32_t)(A->tape_seg * A->K);
         if ((rc = adj_run_tape(c, A, t0, A->tape_len, A->tape_len + todo))) break;
         A->tape_len += todo;
         done += todo;
@@ -303,60 +379,77 @@ static int launch_elem_adj(snh_context* c, const AdjArgs& aa) {
     return SNH_OK;
 }
 
-extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, const double* vel_bar, double* grad) {
-    Range range_("snh_adjoint_reverse");
-    if (!c || !grad) return set_err(SNH_EINVAL, "null argument");
-    AdjointCtx* A = c->adj;
-    if (!A) return set_err(SNH_ESTATE, "snh_adjoint_reverse before snh_adjoint_begin");
-    if (A->steps == 0) return set_err(SNH_ESTATE, "no recorded steps");
-    const size_t E = c->E, N = c->N;
-    const int K = A->K;
-    /* seeds: node-major [N][3] -> [3][N] */
-    std::vector<double> seed(3 * N, 0.0);
-    HIPCHK(hipMemsetAsync(A->sbar, 0, 60 * E * sizeof(double), c->stream));
-    if (A->psbar) HIPCHK(hipMemsetAsync(A->psbar, 0, 10 * E * sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(A->grad, 0, E * sizeof(double), c->stream));
-    /* components 6..8 of nb are written only where a Maxwell law is compiled in */
-    HIPCHK(hipMemsetAsync(A->nb, 0, E * 8 * NBC * sizeof(double), c->stream));
-    for (int which = 0; which < 2; which++) {
-        const double* src = which == 0 ? coord_bar : vel_bar;
-        double* dst = which == 0 ? A->xbar : A->vbar;
-        if (src) {
-            for (size_t n = 0; n < N; n++)
-                for (int d = 0; d < 3; d++) seed[d * N + n] = src[3 * n + d];
-            HIPCHK(hipMemcpyAsync(dst, seed.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));   /* seed is reused */
-        } else {
-            HIPCHK(hipMemsetAsync(dst, 0, 3 * N * sizeof(double), c->stream));
+/* blocks advanced by the binomial schedule to reverse n blocks from a
+   kept state with s snapshots free: t n - C(s + 1 + t, t - 1), t the least
+   integer with C(s + 1 + t, t) >= n (Griewank's count with s + 1 states) */
+static int64_t binomial_cost(int64_t n, int64_t s) {
+    if (n <= 1) return 0;
+    const double S = (double)(s + 1);
+    int64_t t = 0;
+    double b = 1.0;   /* C(S + t, t) */
+    while (b < (double)n) {
+        t++;
+        b = b * (S + (double)t) / (double)t;
+    }
+    return t * n - (int64_t)llround(b * (double)t / (S + 1.0));
+}
+
+/* the split m of [a, a + n) with s snapshots free: advance m blocks, keep
+   that state, reverse the n - m blocks after it with s - 1, then the m
+   before it with s */
+static int64_t binomial_split(int64_t n, int64_t s) {
+    int64_t best = 1, cost = -1;
+    for (int64_t m = 1; m < n; m++) {
+        const int64_t v = m + binomial_cost(n - m, s - 1) + binomial_cost(m, s);
+        if (cost < 0 || v < cost) {
+            cost = v;
+            best = m;
         }
     }
-    const int nseg = (A->steps + K - 1) / K;
-    if ((int)A->ckpt.size() != nseg) return set_err(SNH_ESTATE, "checkpoint count mismatch");
-    std::vector<hipEvent_t> ev;
-    for (int i = 0; i < 3 * nseg; i++) {
+    return best;
+}
+
+/* one reverse sweep: tapes, advances and reverses of blocks */
+struct AdjSweep {
+    snh_context* c;
+    AdjointCtx* A;
+    std::vector<std::pair<hipEvent_t, int>> marks;   /* (event, 0 replay / 1 reverse / 2 end) */
+    bool have_nb;
+
+    int len(int blk) const { return std::min(A->K, A->steps - blk * A->K); }
+    uint32_t t0(int blk) const { return A->ckpt[0].timeStep + (uint32_t)(blk * A->K); }
+    void mark(int kind) {
         hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        ev.push_back(e);
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, c->stream);
+        marks.push_back({e, kind});
     }
-    bool have_nb = false;
-    int rc = SNH_OK;
-    for (int seg = nseg - 1; seg >= 0 && rc == SNH_OK; seg--) {
-        const int s0 = seg * K;
-        const int L = std::min(K, A->steps - s0);
-        const AdjCheckpoint& ck = A->ckpt[seg];
-        (void)hipEventRecord(ev[3 * seg], c->stream);
-        /* the tape still holds the last segment recorded; others are
-           replayed from their checkpoint */
-        if (!(A->tape_seg == seg && A->tape_len == L)) {
-            rc = adj_copy_state(c, A->ts, A->tp, A->tx + 3 * N, A->tx, A->tv, A->swork, ck.stress, ck.ps, ck.x,
-                                ck.xp, ck.v, ck.scal);
-            if (rc == SNH_OK) rc = adj_run_tape(c, A, ck.timeStep, 0, L);
-            A->tape_seg = seg;
-            A->tape_len = L;
+    /* tape slot 0 holds the state entering blk: replay the block into the tape */
+    int tape(int blk) {
+        const int L = len(blk);
+        const int rc = adj_run_tape(c, A, t0(blk), 0, L);
+        A->replay_steps += L;
+        A->tape_seg = blk;
+        A->tape_len = L;
+        return rc;
+    }
+    /* tape slot 0 holds the state entering block `from`: replay blocks up to
+       `to`, after which slot 0 holds the state entering `to` */
+    int advance(int from, int to) {
+        int rc = SNH_OK;
+        for (int b = from; b < to && rc == SNH_OK; b++) {
+            rc = tape(b);
+            if (rc == SNH_OK) rc = adj_roll(c, A, len(b));
         }
-        (void)hipEventRecord(ev[3 * seg + 1], c->stream);
-        /* reverse */
-        for (int i = L - 1; i >= 0 && rc == SNH_OK; i--) {
+        A->tape_seg = -1;
+        A->tape_len = 0;
+        return rc;
+    }
+    /* reverse the steps of blk, whose replay the tape holds */
+    int reverse(int blk) {
+        const size_t E = c->E, N = c->N;
+        int rc = SNH_OK;
+        for (int i = len(blk) - 1; i >= 0 && rc == SNH_OK; i--) {
             NodeAdjArgs na;
             na.nb = have_nb ? A->nb : nullptr;
             na.xbar = A->xbar;
@@ -395,8 +488,94 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
             rc = launch_elem_adj(c, aa);
             have_nb = true;
         }
-        (void)hipEventRecord(ev[3 * seg + 2], c->stream);
+        return rc;
     }
+    /* reverse blocks [a, b) from ck, the state entering a, with the
+       snapshots whose indices pool holds (given back on return) */
+    int revolve(int a, int b, const AdjCheckpoint& ck, std::vector<int>& pool) {
+        int rc = SNH_OK;
+        if (b - a == 1 || pool.empty()) {
+            for (int j = b - 1; j >= a && rc == SNH_OK; j--) {
+                mark(0);
+                rc = adj_load(c, A, ck);
+                if (rc == SNH_OK) rc = advance(a, j);
+                if (rc == SNH_OK) rc = tape(j);
+                mark(1);
+                if (rc == SNH_OK) rc = reverse(j);
+            }
+            return rc;
+        }
+        const int m = (int)binomial_split(b - a, (int64_t)pool.size());
+        const int slot = pool.back();
+        pool.pop_back();
+        mark(0);
+        rc = adj_load(c, A, ck);
+        if (rc == SNH_OK) rc = advance(a, a + m);
+        if (rc == SNH_OK) rc = adj_store(c, A, A->snap[slot], t0(a + m));
+        if (rc == SNH_OK) rc = revolve(a + m, b, A->snap[slot], pool);
+        pool.push_back(slot);
+        if (rc == SNH_OK) rc = revolve(a, a + m, ck, pool);
+        return rc;
+    }
+};
+
+extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, const double* vel_bar, double* grad) {
+    Range range_("snh_adjoint_reverse");
+    if (!c || !grad) return set_err(SNH_EINVAL, "null argument");
+    AdjointCtx* A = c->adj;
+    if (!A) return set_err(SNH_ESTATE, "snh_adjoint_reverse before snh_adjoint_begin");
+    if (A->steps == 0) return set_err(SNH_ESTATE, "no recorded steps");
+    const size_t E = c->E, N = c->N;
+    const int K = A->K;
+    /* seeds: node-major [N][3] -> [3][N] */
+    std::vector<double> seed(3 * N, 0.0);
+    HIPCHK(hipMemsetAsync(A->sbar, 0, 60 * E * sizeof(double), c->stream));
+    if (A->psbar) HIPCHK(hipMemsetAsync(A->psbar, 0, 10 * E * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(A->grad, 0, E * sizeof(double), c->stream));
+    /* components 6..8 of nb are written only where a Maxwell law is compiled in */
+    HIPCHK(hipMemsetAsync(A->nb, 0, E * 8 * NBC * sizeof(double), c->stream));
+    for (int which = 0; which < 2; which++) {
+        const double* src = which == 0 ? coord_bar : vel_bar;
+        double* dst = which == 0 ? A->xbar : A->vbar;
+        if (src) {
+            for (size_t n = 0; n < N; n++)
+                for (int d = 0; d < 3; d++) seed[d * N + n] = src[3 * n + d];
+            HIPCHK(hipMemcpyAsync(dst, seed.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));   /* seed is reused */
+        } else {
+            HIPCHK(hipMemsetAsync(dst, 0, 3 * N * sizeof(double), c->stream));
+        }
+    }
+    const int nseg = (A->steps + K - 1) / K;
+    const bool binomial = !A->snap.empty();
+    if ((int)A->ckpt.size() != (binomial ? 1 : nseg)) return set_err(SNH_ESTATE, "checkpoint count mismatch");
+    AdjSweep sw{c, A, {}, false};
+    A->reversed = true;
+    int rc = SNH_OK;
+    if (!binomial) {
+        for (int seg = nseg - 1; seg >= 0 && rc == SNH_OK; seg--) {
+            sw.mark(0);
+            /* the tape still holds the last block recorded; others are
+               replayed from their checkpoint */
+            if (!(A->tape_seg == seg && A->tape_len == sw.len(seg))) {
+                rc = adj_load(c, A, A->ckpt[seg]);
+                if (rc == SNH_OK) rc = sw.tape(seg);
+            }
+            sw.mark(1);
+            if (rc == SNH_OK) rc = sw.reverse(seg);
+        }
+    } else {
+        int nrev = nseg;
+        if (A->tape_seg == nseg - 1 && A->tape_len == sw.len(nseg - 1)) {
+            sw.mark(1);
+            rc = sw.reverse(nseg - 1);
+            nrev = nseg - 1;
+        }
+        std::vector<int> pool;
+        for (int k = (int)A->snap.size() - 1; k >= 0; k--) pool.push_back(k);
+        if (rc == SNH_OK && nrev > 0) rc = sw.revolve(0, nrev, A->ckpt[0], pool);
+    }
+    sw.mark(2);
     if (rc == SNH_OK) {
         if (hipMemcpyAsync(grad, A->grad, E * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
@@ -405,15 +584,14 @@ extern "C" int snh_adjoint_reverse(snh_context* c, const double* coord_bar, cons
         (void)hipStreamSynchronize(c->stream);
     }
     if (rc == SNH_OK) {
-        for (int seg = 0; seg < nseg; seg++) {
-            float a = 0.0f, b = 0.0f;
-            (void)hipEventElapsedTime(&a, ev[3 * seg], ev[3 * seg + 1]);
-            (void)hipEventElapsedTime(&b, ev[3 * seg + 1], ev[3 * seg + 2]);
-            A->replay_ms += a;
-            A->reverse_ms += b;
+        /* marks: 0 opens a replay, 1 a reverse, 2 closes the last phase */
+        for (size_t k = 0; k + 1 < sw.marks.size(); k++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, sw.marks[k].first, sw.marks[k + 1].first);
+            (sw.marks[k].second == 0 ? A->replay_ms : A->reverse_ms) += ms;
         }
     }
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (auto& m : sw.marks) (void)hipEventDestroy(m.first);
     return rc;
 }
 
@@ -425,7 +603,8 @@ extern "C" int snh_adjoint_get_times(snh_context* c, snh_adjoint_times* out) {
     out->replay_ms = A->replay_ms;
     out->reverse_ms = A->reverse_ms;
     out->steps = A->steps;
-    out->checkpoints = (int32_t)A->ckpt.size();
+    out->checkpoints = (int32_t)(A->snap.empty() ? A->ckpt.size() : A->snap.size() + 1);
+    out->replay_steps = A->replay_steps;
     return SNH_OK;
 }
 

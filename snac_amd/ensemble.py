@@ -58,7 +58,7 @@ class Ensemble:
 
 
 def adjoint_gradients(members: Sequence[Context], param: int, nsteps: int, targets: Sequence[np.ndarray],
-                      checkpoint_interval: int = 50, flags: int = 0, threads: int = 0):
+                      checkpoint_interval: int = 50, flags: int = 0, threads: int = 0, snapshots: int = 0):
     """For each member (initialised contexts, not yet stepped): the misfit
     J = 1/2 sum over top nodes of (uplift - target)^2 after nsteps steps and
     its gradient with respect to the per-element parameter (`param`, an
@@ -70,7 +70,7 @@ def adjoint_gradients(members: Sequence[Context], param: int, nsteps: int, targe
         c = members[k]
         top = top_nodes(c.problem)
         x0 = c.state(fields=("coord",))["coord"]
-        c.adjoint_begin(param, checkpoint_interval, flags)
+        c.adjoint_begin(param, checkpoint_interval, flags, snapshots)
         c.adjoint_forward(nsteps)
         x = c.state(fields=("coord",))["coord"]
         r = (x[top, 1] - x0[top, 1]) - np.asarray(targets[k], dtype=np.float64)
