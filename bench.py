@@ -145,8 +145,11 @@ def adjoint_bench(make, param, steps, interval, flags=0):
 
 
 def _ensemble_times(probs, steps, graph_steps):
-    """Seconds for `steps` steps of the members in graph launches, then
-    stepped one after the other, and the autotuned element variants."""
+    """Seconds for `steps` steps of the members in graph launches; then with
+    per-member launches from one host thread, each member on its own stream
+    (members overlap on the GPU as far as its hardware queues allow); then
+    serialised, each member's steps finished before the next member starts;
+    and the autotuned element variants."""
     from snac_amd import Context
     from snac_amd.ensemble import Ensemble
     members = [Context(p).Construct().Build().Initialise() for p in probs]
@@ -163,32 +166,40 @@ def _ensemble_times(probs, steps, graph_steps):
     for m in members:
         m.synchronize()
     t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for m in members:
+        m.step(steps)
+        m.synchronize()
+    t_ser = time.perf_counter() - t0
     for m in members:
         m.Destroy()
-    return t_ens, t_seq, variants
+    return t_ens, t_seq, t_ser, variants
 
 
 def ensemble_bench(n_models, steps, graph_steps=10):
     """Aggregate element-steps/s of an ensemble of uneven elasto-plastic
     models: one graph launch per graph_steps steps (members on parallel
-    branches) against the same members stepped one after the other; FP64,
-    then the same members with FP32 bulk arithmetic and FP64 state."""
+    branches) against the same members launched member by member (on their
+    own streams, and serialised); FP64, then the same members with FP32 bulk
+    arithmetic and FP64 state."""
     import copy
     from snac_amd import _abi
     from snac_amd.ensemble import elastoplastic_ensemble
     probs = elastoplastic_ensemble(n_models=n_models)
     total_e = sum(p.n_elements for p in probs)
-    t_ens, t_seq, variants = _ensemble_times(probs, steps, graph_steps)
+    t_ens, t_seq, t_ser, variants = _ensemble_times(probs, steps, graph_steps)
     out = {"models": n_models, "elements_total": total_e, "steps": steps, "graph_steps": graph_steps,
            "value": total_e * steps / t_ens, "unit": "element-steps/s",
            "sequential_value": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_ens,
+           "serialised_value": total_e * steps / t_ser, "speedup_vs_serialised": t_ser / t_ens,
            "elem_variants": {"tet_per_lane": variants.count(0), "element_per_lane": variants.count(1),
                              "element_per_lane_lowreg": variants.count(2), "tet_per_wave": variants.count(3)}}
     mixed = [copy.deepcopy(p) for p in probs]
     for p in mixed:
         p.precision = _abi.PRECISION_MIXED
-    m_ens, m_seq, m_variants = _ensemble_times(mixed, steps, graph_steps)
+    m_ens, m_seq, m_ser, m_variants = _ensemble_times(mixed, steps, graph_steps)
     out["mixed"] = {"value": total_e * steps / m_ens, "sequential_value": total_e * steps / m_seq,
+                    "serialised_value": total_e * steps / m_ser,
                     "vs_fp64": t_ens / m_ens,
                     "elem_variants": {"tet_per_lane": m_variants.count(0),
                                       "tet_per_wave": len(m_variants) - m_variants.count(0)}}
@@ -196,10 +207,11 @@ def ensemble_bench(n_models, steps, graph_steps=10):
     return out
 
 
-def ensemble_adjoint_bench(probs, steps, interval=10):
+def ensemble_adjoint_bench(probs, steps, interval=25, snapshots=8):
     """Forward + adjoint of every member (cohesion gradient of a top-surface
-    misfit, corner clamp): members on their own streams from one host thread
-    each, against the members run one after the other."""
+    misfit, corner clamp, binomial checkpointing: blocks of `interval` steps,
+    `snapshots` states beside the first): members on their own streams from
+    one host thread each, against the members run one after the other."""
     import numpy as np
     from snac_amd import Context, _abi
     from snac_amd.autograd import top_nodes
@@ -209,7 +221,8 @@ def ensemble_adjoint_bench(probs, steps, interval=10):
     total_e = sum(p.n_elements for p in probs)
     members = [Context(p).Construct().Build().Initialise() for p in probs]
     t0 = time.perf_counter()
-    res = adjoint_gradients(members, _abi.PARAM_COHESION, steps, targets, interval, _abi.ADJ_CORNER_CLAMP)
+    res = adjoint_gradients(members, _abi.PARAM_COHESION, steps, targets, interval, _abi.ADJ_CORNER_CLAMP,
+                            snapshots=snapshots)
     t_con = time.perf_counter() - t0
     for m in members:
         m.Destroy()
@@ -217,10 +230,12 @@ def ensemble_adjoint_bench(probs, steps, interval=10):
     for k, p in enumerate(probs):
         c = Context(p).Construct().Build().Initialise()
         t0 = time.perf_counter()
-        adjoint_gradients([c], _abi.PARAM_COHESION, steps, [targets[k]], interval, _abi.ADJ_CORNER_CLAMP, threads=1)
+        adjoint_gradients([c], _abi.PARAM_COHESION, steps, [targets[k]], interval, _abi.ADJ_CORNER_CLAMP, threads=1,
+                          snapshots=snapshots)
         t_seq += time.perf_counter() - t0
         c.Destroy()
-    return {"steps": steps, "checkpoint_interval": interval,
+    return {"steps": steps, "checkpoint_interval": interval, "snapshots": snapshots,
+            "state_bytes_per_element": (snapshots + 1) * 560 + interval * 750,
             "forward_plus_adjoint_element_steps_per_s": total_e * steps / t_con,
             "sequential_element_steps_per_s": total_e * steps / t_seq, "speedup_vs_sequential": t_seq / t_con,
             "finite_gradients": all(bool(np.all(np.isfinite(g))) for _, g in res)}
@@ -264,7 +279,39 @@ def mixed_c3_bench(steps=200, warmup=10):
             "error_vs_fp64": {"topography": rel(b["uplift"], a["uplift"]),
                               "stress": rel(b["st"]["stress"], a["st"]["stress"]),
                               "velocity": rel(b["st"]["velocity"], a["st"]["velocity"])},
+            "error_on_fp64_schedule": mixed_error_on_schedule(steps + warmup),
             "elem_variants": {"fp64": a["variant"], "mixed": b["variant"]}}
+
+
+def mixed_error_on_schedule(steps, seg=50):
+    """The mixed run's arithmetic error alone: the mixed context replays the
+    FP64 run's own dt, speed of sound and damping branch of every node
+    component (snh_record_schedule / snh_set_schedule, in segments of `seg`
+    steps), so that no discrete decision of the damped map differs; relative
+    to the FP64 run's largest uplift, stress and velocity."""
+    import numpy as np
+    from snac_amd import Context, _abi
+    from snac_amd import problem as P
+    from snac_amd.autograd import top_nodes
+    ctx = {}
+    for prec in ("fp64", "mixed"):
+        p = P.maxwell_viscoelastic()
+        p.precision = _abi.PRECISION_MIXED if prec == "mixed" else _abi.PRECISION_FP64
+        ctx[prec] = Context(p).Construct().Build().Initialise()
+    x0 = ctx["fp64"].state(fields=("coord",))["coord"]
+    for s0 in range(0, steps, seg):
+        n = min(seg, steps - s0)
+        ctx["mixed"].set_schedule(*ctx["fp64"].record_schedule(n, damping=True))
+        ctx["mixed"].Execute(n)
+    a = ctx["fp64"].state(fields=("coord", "velocity", "stress"))
+    b = ctx["mixed"].state(fields=("coord", "velocity", "stress"))
+    for c in ctx.values():
+        c.Destroy()
+    top = top_nodes(p)
+    rel = lambda x, y: float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+    return {"steps": steps, "topography": rel(b["coord"][top, 1] - x0[top, 1], a["coord"][top, 1] - x0[top, 1]),
+            "displacement": rel(b["coord"] - x0, a["coord"] - x0),
+            "stress": rel(b["stress"], a["stress"]), "velocity": rel(b["velocity"], a["velocity"])}
 
 
 def _pmc_child_cmd(args):
@@ -381,7 +428,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-model ensemble line")
-    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs behind roofline.traffic")
     ap.add_argument("--no-mixed-c3", action="store_true", help="skip the mixed-precision Maxwell line")

@@ -459,9 +459,23 @@ int snh_adjoint_end(snh_context* ctx);
  * advances every member graph_steps steps with no host work in between.
  * Results are bit-identical to stepping each member with snh_step.
  * graph_steps must be even (the coordinate buffers alternate every step);
- * members must have no schedule and profiling off while in an ensemble. */
+ * members must have no schedule and profiling off while in an ensemble.
+ *
+ * Batched launches.  When the members share one precision and one single
+ * law (elastic, Mohr-Coulomb or Maxwell), the same material layout, the
+ * per-node node pass and no dump diagnostics or tet densities, the graph
+ * instead holds, per step, one launch of each pass over all members' blocks
+ * (the element pass in the tet-per-lane layout whatever the members'
+ * autotuned variants, which give the same bits): 4 launches per step for
+ * the whole ensemble instead of 4 per member.  Environment at creation:
+ * SNH_ENSEMBLE_BATCH=0 keeps the per-member branches; SNH_ENSEMBLE_TW=1
+ * takes the tet-per-wave layout; SNH_ENSEMBLE_GROUPS=k splits the members
+ * into k groups of about equal size, batched on k streams. */
 typedef struct snh_ensemble snh_ensemble;
 int snh_ensemble_create(snh_context* const* members, int32_t n, int32_t graph_steps, snh_ensemble** out);
+/* 1 when the ensemble steps its members by batched launches, 0 when by
+ * per-member branches; negative on error. */
+int snh_ensemble_batched(const snh_ensemble* ens);
 /* nsteps steps of every member: whole graph launches, the rest with snh_step */
 int snh_ensemble_step(snh_ensemble* ens, int32_t nsteps);
 int snh_ensemble_synchronize(snh_ensemble* ens);

@@ -216,6 +216,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_int8)]),
     "snh_ensemble_create": (_I, [C.POINTER(_P), C.c_int32, C.c_int32, C.POINTER(_P)]),
     "snh_ensemble_step": (_I, [_P, C.c_int32]),
+    "snh_ensemble_batched": (_I, [_P]),
     "snh_ensemble_synchronize": (_I, [_P]),
     "snh_ensemble_destroy": (_I, [_P]),
     "snh_adjoint_begin": (_I, [_P, C.POINTER(AdjointConfig)]),

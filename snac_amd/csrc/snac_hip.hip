@@ -628,7 +628,7 @@ struct Soft {
    taking tet w of all of them, so every stress and node load of a wave is
    over 64 consecutive elements. */
 template <int RSET, bool TW, bool MM>
-__global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
+__device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int nblk) {
     constexpr int NE = TW ? 64 : EPW;       /* elements per block */
     constexpr int NT = TW ? NTET + 1 : NTET; /* padded against LDS bank conflicts */
     __shared__ double sh_a[NE][NT];
@@ -641,7 +641,7 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
     const bool in_wave = TW || lane < EPW * NTET;
     const int le = TW ? (lane & 63) : (in_wave ? lane / NTET : 0);
     const int t = TW ? (lane >> 6) : (in_wave ? lane - le * NTET : 0);
-    const int e = xcd_block() * NE + le;
+    const int e = xcd_block(blk, nblk) * NE + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
@@ -846,6 +846,11 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
         for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + ec] = st[c];
         if (a.tpre) store_density(a, m, nd, t, ec);
     }
+}
+
+template <int RSET, bool TW, bool MM>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem(StepArgs a) {
+    elem_pass<RSET, TW, MM>(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
 /* --------------------------------------------------------------------------
@@ -1306,9 +1311,9 @@ __device__ __forceinline__ double node_finish(const StepArgs& a, int n, double f
 }
 
 template <bool MM, bool TH>
-__global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
+__device__ __forceinline__ void node_pass(StepArgs a, const int blk, const int nblk) {
     __shared__ double sh[NODE_BLOCK / 64];
-    const int g = xcd_block() * blockDim.x + threadIdx.x;
+    const int g = xcd_block(blk, nblk) * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
     if (g < a.N) {
         const int n = gather_node(a, g);
@@ -1373,7 +1378,12 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
         vmag = node_finish(a, n, f0, f1, f2, im, a.xc[n], a.xc[(size_t)a.N + n], a.xc[2 * (size_t)a.N + n], dt);
     }
     double bm = block_max(vmag, sh);
-    if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+    if (threadIdx.x == 0) a.part_vmax[blk] = bm;
+}
+
+template <bool MM, bool TH>
+__global__ void __launch_bounds__(NODE_BLOCK) k_node(StepArgs a) {
+    node_pass<MM, TH>(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
 /* --------------------------------------------------------------------------
@@ -2170,9 +2180,9 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_heat_node(StepArgs a) {
 /* --------------------------------------------------------------------------
  * k_geo: Snac_EP_LoopElementsMomentum's geometry and minLengthScale
  * ------------------------------------------------------------------------ */
-__global__ void __launch_bounds__(GEO_BLOCK) k_geo(StepArgs a) {
+__device__ __forceinline__ void geo_pass(StepArgs a, const int blk, const int nblk) {
     __shared__ double sh[GEO_BLOCK / 64];
-    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    const int e = xcd_block(blk, nblk) * blockDim.x + threadIdx.x;
     double emin = INFINITY;
     if (e < a.E) {
         const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
@@ -2212,15 +2222,17 @@ __global__ void __launch_bounds__(GEO_BLOCK) k_geo(StepArgs a) {
     }
     double bm = block_min(emin, sh);
     if (threadIdx.x == 0) {
-        a.part_min[blockIdx.x] = bm;
-        if (blockIdx.x == 0) a.scal->dt_rot = a.scal->dt;
+        a.part_min[blk] = bm;
+        if (blk == 0) a.scal->dt_rot = a.scal->dt;
     }
 }
+
+__global__ void __launch_bounds__(GEO_BLOCK) k_geo(StepArgs a) { geo_pass(a, (int)blockIdx.x, (int)gridDim.x); }
 
 /* --------------------------------------------------------------------------
  * k_fin: _Snac_Context_Sync
  * ------------------------------------------------------------------------ */
-__global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvmax) {
+__device__ __forceinline__ void fin_pass(StepArgs a, const int nmin, const int nvmax) {
     __shared__ double sh[FIN_BLOCK / 64];
     double mn = INFINITY, mx = -INFINITY;
     for (int q = threadIdx.x; q < nmin; q += blockDim.x) mn = fmin(mn, a.part_min[q]);
@@ -2243,6 +2255,180 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvm
         }
         if (!isfinite(s->dt)) s->nonfinite = 1;
     }
+}
+
+__global__ void __launch_bounds__(FIN_BLOCK) k_fin(StepArgs a, int nmin, int nvmax) { fin_pass(a, nmin, nvmax); }
+
+/* --------------------------------------------------------------------------
+ * Batched launches of an ensemble: one grid over the blocks of several
+ * contexts.  Block b of the grid belongs to member member[b], as that
+ * member's block b - first[member]; first[] is a multiple of 8, so every
+ * member's blocks keep the XCD numbering they have in a launch of their own
+ * (xcd_block), and the blocks past a member's count (padding to the next
+ * multiple of 8) return at once.  Each member's pass is the one of its own
+ * launch, on its own StepArgs, so the results are the same bits.
+ * ------------------------------------------------------------------------ */
+/* A member's StepArgs relative to the launch's first member: each pointer
+   as a byte offset from that member's (null pointers included: offsets are
+   integer differences), so that the kernel forms it from a kernel argument
+   and the compiler keeps global-memory loads (a pointer read from memory
+   would be a generic one, with flat loads); the sizes and constants as
+   they are. */
+struct BatchMember {
+    int64_t xc;
+    int64_t xp;
+    int64_t xn;
+    int64_t v;
+    int64_t v_out;
+    int64_t stress;
+    int64_t stress_out;
+    int64_t ps;
+    int64_t ps_out;
+    int64_t visc;
+    int64_t mat;
+    int64_t refvisc;
+    int64_t cohoff;
+    int64_t bc;
+    int64_t tape_f;
+    int64_t tape_m;
+    int64_t tape_code;
+    int64_t damp_sched;
+    int64_t diag;
+    int64_t force_out;
+    int64_t temp;
+    int64_t temp0;
+    int64_t tpre;
+    int64_t hflux;
+    int64_t tstress;
+    int64_t dens;
+    int64_t tbc;
+    int64_t tbcv;
+    int64_t mats;
+    int64_t bcs;
+    int64_t part_min;
+    int64_t part_vmax;
+    int64_t scal;
+    int32_t I, J, K, EI, EJ, EK, N, E, dtType, node_band, node_kc;
+    double gravity, demf, strain_inert, c_wave;
+};
+
+struct BatchGrid {
+    StepArgs base;                          /* the first member, this step's parity */
+    const BatchMember* __restrict__ mem;    /* [members], same parity               */
+    const int* __restrict__ first;          /* [members] first block                */
+    const int* __restrict__ count;          /* [members] blocks of the member       */
+    const uint16_t* __restrict__ member;    /* [grid] member of each block          */
+};
+
+__device__ __forceinline__ StepArgs member_args(const StepArgs& base, const BatchMember* __restrict__ mem,
+                                                const int m) {
+    StepArgs a = base;
+    const BatchMember& d = mem[m];
+#define SNH_REBASE(f) a.f = (decltype(a.f))((const char*)base.f + d.f)
+    SNH_REBASE(xc);
+    SNH_REBASE(xp);
+    SNH_REBASE(xn);
+    SNH_REBASE(v);
+    SNH_REBASE(v_out);
+    SNH_REBASE(stress);
+    SNH_REBASE(stress_out);
+    SNH_REBASE(ps);
+    SNH_REBASE(ps_out);
+    SNH_REBASE(visc);
+    SNH_REBASE(mat);
+    SNH_REBASE(refvisc);
+    SNH_REBASE(cohoff);
+    SNH_REBASE(bc);
+    SNH_REBASE(tape_f);
+    SNH_REBASE(tape_m);
+    SNH_REBASE(tape_code);
+    SNH_REBASE(damp_sched);
+    SNH_REBASE(diag);
+    SNH_REBASE(force_out);
+    SNH_REBASE(temp);
+    SNH_REBASE(temp0);
+    SNH_REBASE(tpre);
+    SNH_REBASE(hflux);
+    SNH_REBASE(tstress);
+    SNH_REBASE(dens);
+    SNH_REBASE(tbc);
+    SNH_REBASE(tbcv);
+    SNH_REBASE(mats);
+    SNH_REBASE(bcs);
+    SNH_REBASE(part_min);
+    SNH_REBASE(part_vmax);
+    SNH_REBASE(scal);
+#undef SNH_REBASE
+    a.I = d.I;
+    a.J = d.J;
+    a.K = d.K;
+    a.EI = d.EI;
+    a.EJ = d.EJ;
+    a.EK = d.EK;
+    a.N = d.N;
+    a.E = d.E;
+    a.dtType = d.dtType;
+    a.node_band = d.node_band;
+    a.node_kc = d.node_kc;
+    a.gravity = d.gravity;
+    a.demf = d.demf;
+    a.strain_inert = d.strain_inert;
+    a.c_wave = d.c_wave;
+    return a;
+}
+
+/* the member of this block (uniform: an SGPR) and its block index */
+__device__ __forceinline__ int batch_member(const BatchGrid& g, int& b) {
+    const int m = __builtin_amdgcn_readfirstlane((int)g.member[blockIdx.x]);
+    b = (int)blockIdx.x - g.first[m];
+    return m;
+}
+
+template <int RSET, bool TW, bool MM>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem_batch(BatchGrid g) {
+    int b;
+    const int m = batch_member(g, b);
+    if (b >= g.count[m]) return;
+    elem_pass<RSET, TW, MM>(member_args(g.base, g.mem, m), b, g.count[m]);
+}
+
+template <int RSET, bool TW>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx_batch(BatchGrid g) {
+    int b;
+    const int m = batch_member(g, b);
+    if (b >= g.count[m]) return;
+    elem_pass_mx<RSET, TW>(member_args(g.base, g.mem, m), b, g.count[m]);
+}
+
+template <bool MM>
+__global__ void __launch_bounds__(NODE_BLOCK) k_node_batch(BatchGrid g) {
+    int b;
+    const int m = batch_member(g, b);
+    if (b >= g.count[m]) return;
+    node_pass<MM, false>(member_args(g.base, g.mem, m), b, g.count[m]);
+}
+
+__global__ void __launch_bounds__(NODE_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_node_mx_batch(BatchGrid g) {
+    int b;
+    const int m = batch_member(g, b);
+    if (b >= g.count[m]) return;
+    node_pass_mx(member_args(g.base, g.mem, m), b, g.count[m]);
+}
+
+__global__ void __launch_bounds__(GEO_BLOCK) k_geo_batch(BatchGrid g) {
+    int b;
+    const int m = batch_member(g, b);
+    if (b >= g.count[m]) return;
+    geo_pass(member_args(g.base, g.mem, m), b, g.count[m]);
+}
+
+/* block m: member m's Sync, over its geo and node block partials */
+__global__ void __launch_bounds__(FIN_BLOCK) k_fin_batch(StepArgs base, const BatchMember* __restrict__ mem,
+                                                         const int* __restrict__ nmin,
+                                                         const int* __restrict__ nvmax) {
+    const int m = (int)blockIdx.x;
+    fin_pass(member_args(base, mem, m), nmin[m], nvmax[m]);
 }
 
 /* minLengthScale after a remesh: the running minimum of
@@ -3927,6 +4113,20 @@ extern "C" int snh_sync_state(const snh_config* cfg, const double* coord, const 
 /* ==========================================================================
  * ensembles: one HIP graph steps every member, members on parallel branches
  * ======================================================================== */
+/* a group of members stepped by batched launches on one stream: the tables
+   of BatchGrid for the element, node and geometry passes */
+struct EnsGroup {
+    std::vector<int> idx;                      /* members of the ensemble       */
+    hipStream_t stream = nullptr;
+    hipEvent_t join = nullptr;
+    StepArgs base[2];                          /* first member's StepArgs, parity q */
+    BatchMember* dmem[2] = {nullptr, nullptr}; /* members relative to it, parity q  */
+    int* dfirst[3] = {nullptr, nullptr, nullptr};
+    int* dcount[3] = {nullptr, nullptr, nullptr};
+    uint16_t* dmember[3] = {nullptr, nullptr, nullptr};
+    int grid[3] = {0, 0, 0};
+};
+
 struct snh_ensemble {
     std::vector<snh_context*> m;
     std::vector<int> cur0;      /* members' coordinate-buffer index at capture */
@@ -3937,7 +4137,16 @@ struct snh_ensemble {
     /* [0]: buffers as at capture, [1]: every member's buffers swapped */
     hipGraph_t graph[2];
     hipGraphExec_t exec[2];
+    /* batched launches (members of one precision and law): the members in
+       groups, each group's passes batched on a stream of its own */
+    bool batched = false;
+    bool tw = false;            /* element pass layout of the batched launches:
+                                   tet per lane (tet per wave with
+                                   SNH_ENSEMBLE_TW=1) */
+    std::vector<EnsGroup> groups;
 };
+
+enum { EB_ELEM = 0, EB_NODE = 1, EB_GEO = 2 };
 
 static void ens_free(snh_ensemble* e) {
     if (!e) return;
@@ -3949,7 +4158,238 @@ static void ens_free(snh_ensemble* e) {
     for (hipEvent_t j : e->join)
         if (j) (void)hipEventDestroy(j);
     if (e->origin) (void)hipStreamDestroy(e->origin);
+    for (EnsGroup& g : e->groups) {
+        for (int q = 0; q < 2; q++)
+            if (g.dmem[q]) (void)hipFree(g.dmem[q]);
+        for (int k = 0; k < 3; k++) {
+            if (g.dfirst[k]) (void)hipFree(g.dfirst[k]);
+            if (g.dcount[k]) (void)hipFree(g.dcount[k]);
+            if (g.dmember[k]) (void)hipFree(g.dmember[k]);
+        }
+        if (g.join) (void)hipEventDestroy(g.join);
+        if (g.stream) (void)hipStreamDestroy(g.stream);
+    }
     delete e;
+}
+
+/* Members that one batched launch per pass can step: one precision, one
+   single law, one material layout (with or without an element material
+   map), the per-node FP64 node pass, no dump diagnostics or tet densities.
+   SNH_ENSEMBLE_BATCH=0 in the environment keeps the per-member graph. */
+static bool ens_batchable(const std::vector<snh_context*>& m) {
+    const char* env = getenv("SNH_ENSEMBLE_BATCH");
+    if (env && env[0] == '0') return false;
+    if (m.size() > 65535) return false;
+    const snh_context* c0 = m[0];
+    if (c0->rset != RS_ELASTIC && c0->rset != RS_PLASTIC && c0->rset != RS_MAXWELL) return false;
+    for (const snh_context* c : m) {
+        if (c->cfg.precision != c0->cfg.precision || c->rset != c0->rset ||
+            (c->mat != nullptr) != (c0->mat != nullptr))
+            return false;
+        if (c->diag_on || c->beta_dens || c->thermal || c->node_variant != SNH_NODE_PER_NODE) return false;
+        if (c->cfg.precision == SNH_PRECISION_MIXED && c->mat) return false;
+    }
+    return true;
+}
+
+/* host -> device copy ordered on the ensemble's stream, complete on return */
+static hipError_t ens_upload(snh_ensemble* e, void* dst, const void* src, size_t bytes) {
+    hipError_t r = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->origin);
+    return r != hipSuccess ? r : hipStreamSynchronize(e->origin);
+}
+
+/* the members in groups of about equal element counts (largest first, each
+   to the lightest group): each group's batched passes run on a stream of
+   its own, so one group's element pass may overlap another's node pass.
+   SNH_ENSEMBLE_GROUPS sets the count (default 1: on the 64-model ensemble
+   one group is the fastest; on hundreds of tiny models four are). */
+static int ens_make_groups(snh_ensemble* e) {
+    const int n = (int)e->m.size();
+    int ng = 1;
+    if (const char* env = getenv("SNH_ENSEMBLE_GROUPS")) ng = atoi(env);
+    if (ng < 1) ng = 1;
+    if (ng > n) ng = n;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->m[a]->E > e->m[b]->E; });
+    e->groups.resize(ng);
+    std::vector<size_t> load(ng, 0);
+    for (int i : order) {
+        int g = 0;
+        for (int k = 1; k < ng; k++)
+            if (load[k] < load[g]) g = k;
+        e->groups[g].idx.push_back(i);
+        load[g] += e->m[i]->E;
+    }
+    for (EnsGroup& g : e->groups) {
+        std::sort(g.idx.begin(), g.idx.end());
+        HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g.join, hipEventDisableTiming));
+    }
+    return SNH_OK;
+}
+
+/* device tables of a group's batched launches; its members' StepArgs for
+   both coordinate-buffer parities, relative to its first member's */
+static int ens_batch_tables(snh_ensemble* e, EnsGroup& grp) {
+    const int n = (int)grp.idx.size();
+    std::vector<int> cnt[3], first[3];
+    for (int k = 0; k < 3; k++) {
+        cnt[k].resize(n);
+        first[k].resize(n);
+    }
+    for (int i = 0; i < n; i++) {
+        const snh_context* c = e->m[grp.idx[i]];
+        cnt[EB_ELEM][i] = (int)(e->tw ? (c->E + 63) / 64 : (c->E + EPW - 1) / EPW);
+        cnt[EB_NODE][i] = c->n_node_blocks;
+        cnt[EB_GEO][i] = c->n_geo_blocks;
+    }
+    for (int k = 0; k < 3; k++) {
+        int64_t at = 0;
+        for (int i = 0; i < n; i++) {
+            first[k][i] = (int)at;
+            at += ((int64_t)cnt[k][i] + 7) / 8 * 8;
+        }
+        if (at > INT32_MAX) return set_err(SNH_EUNSUPPORTED, "ensemble too large for one batched launch");
+        grp.grid[k] = (int)at;
+        std::vector<uint16_t> mem((size_t)at);
+        for (int i = 0; i < n; i++)
+            for (int64_t b = first[k][i]; b < first[k][i] + ((int64_t)cnt[k][i] + 7) / 8 * 8; b++)
+                mem[(size_t)b] = (uint16_t)i;
+        HIPCHK(hipMalloc((void**)&grp.dfirst[k], n * sizeof(int)));
+        HIPCHK(hipMalloc((void**)&grp.dcount[k], n * sizeof(int)));
+        HIPCHK(hipMalloc((void**)&grp.dmember[k], mem.size() * sizeof(uint16_t)));
+        HIPCHK(ens_upload(e, grp.dfirst[k], first[k].data(), n * sizeof(int)));
+        HIPCHK(ens_upload(e, grp.dcount[k], cnt[k].data(), n * sizeof(int)));
+        HIPCHK(ens_upload(e, grp.dmember[k], mem.data(), mem.size() * sizeof(uint16_t)));
+    }
+    for (int q = 0; q < 2; q++) {
+        std::vector<StepArgs> args(n);
+        for (int i = 0; i < n; i++) {
+            snh_context* c = e->m[grp.idx[i]];
+            const int cur = c->cur;
+            c->cur = e->cur0[grp.idx[i]] ^ q;
+            args[i] = make_args(c);
+            c->cur = cur;
+        }
+        const StepArgs& b = args[0];
+        grp.base[q] = b;
+        std::vector<BatchMember> mem(n);
+        for (int i = 0; i < n; i++) {
+            const StepArgs& a = args[i];
+            BatchMember& d = mem[i];
+#define SNH_DELTA(f) d.f = (int64_t)((intptr_t)a.f - (intptr_t)b.f)
+            SNH_DELTA(xc);
+            SNH_DELTA(xp);
+            SNH_DELTA(xn);
+            SNH_DELTA(v);
+            SNH_DELTA(v_out);
+            SNH_DELTA(stress);
+            SNH_DELTA(stress_out);
+            SNH_DELTA(ps);
+            SNH_DELTA(ps_out);
+            SNH_DELTA(visc);
+            SNH_DELTA(mat);
+            SNH_DELTA(refvisc);
+            SNH_DELTA(cohoff);
+            SNH_DELTA(bc);
+            SNH_DELTA(tape_f);
+            SNH_DELTA(tape_m);
+            SNH_DELTA(tape_code);
+            SNH_DELTA(damp_sched);
+            SNH_DELTA(diag);
+            SNH_DELTA(force_out);
+            SNH_DELTA(temp);
+            SNH_DELTA(temp0);
+            SNH_DELTA(tpre);
+            SNH_DELTA(hflux);
+            SNH_DELTA(tstress);
+            SNH_DELTA(dens);
+            SNH_DELTA(tbc);
+            SNH_DELTA(tbcv);
+            SNH_DELTA(mats);
+            SNH_DELTA(bcs);
+            SNH_DELTA(part_min);
+            SNH_DELTA(part_vmax);
+            SNH_DELTA(scal);
+#undef SNH_DELTA
+            d.I = a.I;
+            d.J = a.J;
+            d.K = a.K;
+            d.EI = a.EI;
+            d.EJ = a.EJ;
+            d.EK = a.EK;
+            d.N = a.N;
+            d.E = a.E;
+            d.dtType = a.dtType;
+            d.node_band = a.node_band;
+            d.node_kc = a.node_kc;
+            d.gravity = a.gravity;
+            d.demf = a.demf;
+            d.strain_inert = a.strain_inert;
+            d.c_wave = a.c_wave;
+        }
+        HIPCHK(hipMalloc((void**)&grp.dmem[q], n * sizeof(BatchMember)));
+        HIPCHK(ens_upload(e, grp.dmem[q], mem.data(), n * sizeof(BatchMember)));
+    }
+    return SNH_OK;
+}
+
+/* one step of a group's members, batched, on stream s, entering with
+   buffer parity q: the element and node passes read the coordinates of
+   parity q, the node pass writes the other buffer, and the geometry pass
+   and Sync see the buffers swapped (as snh_step's launches do after the
+   node pass flips the context's index) */
+static int ens_batch_step(const snh_ensemble* e, const EnsGroup& grp, hipStream_t s, int q) {
+    const snh_context* c0 = e->m[grp.idx[0]];
+    const bool mm = c0->mat != nullptr;
+    BatchGrid g[3];
+    for (int k = 0; k < 3; k++) {
+        const int p = k == EB_GEO ? q ^ 1 : q;
+        g[k] = BatchGrid{grp.base[p], grp.dmem[p], grp.dfirst[k], grp.dcount[k], grp.dmember[k]};
+    }
+    const dim3 ge(grp.grid[EB_ELEM]), gn(grp.grid[EB_NODE]), gg(grp.grid[EB_GEO]);
+    const dim3 be(e->tw ? 640 : 64);
+    const bool tw = e->tw;
+    if (c0->cfg.precision == SNH_PRECISION_MIXED) {
+#define EB_MX_LAUNCH(R)                                                                          \
+    do {                                                                                         \
+        if (tw) hipLaunchKernelGGL((k_elem_mx_batch<R, true>), ge, be, 0, s, g[EB_ELEM]);        \
+        else hipLaunchKernelGGL((k_elem_mx_batch<R, false>), ge, be, 0, s, g[EB_ELEM]);          \
+    } while (0)
+        switch (c0->rset) {
+        case RS_ELASTIC: EB_MX_LAUNCH(RS_ELASTIC); break;
+        case RS_PLASTIC: EB_MX_LAUNCH(RS_PLASTIC); break;
+        default: EB_MX_LAUNCH(RS_MAXWELL); break;
+        }
+#undef EB_MX_LAUNCH
+        hipLaunchKernelGGL(k_node_mx_batch, gn, dim3(NODE_BLOCK), 0, s, g[EB_NODE]);
+    } else {
+#define EB_ELEM_LAUNCH(R)                                                                        \
+    do {                                                                                         \
+        if (mm) {                                                                                \
+            if (tw) hipLaunchKernelGGL((k_elem_batch<R, true, true>), ge, be, 0, s, g[EB_ELEM]); \
+            else hipLaunchKernelGGL((k_elem_batch<R, false, true>), ge, be, 0, s, g[EB_ELEM]);   \
+        } else {                                                                                 \
+            if (tw) hipLaunchKernelGGL((k_elem_batch<R, true, false>), ge, be, 0, s, g[EB_ELEM]); \
+            else hipLaunchKernelGGL((k_elem_batch<R, false, false>), ge, be, 0, s, g[EB_ELEM]);  \
+        }                                                                                        \
+    } while (0)
+        switch (c0->rset) {
+        case RS_ELASTIC: EB_ELEM_LAUNCH(RS_ELASTIC); break;
+        case RS_PLASTIC: EB_ELEM_LAUNCH(RS_PLASTIC); break;
+        default: EB_ELEM_LAUNCH(RS_MAXWELL); break;
+        }
+#undef EB_ELEM_LAUNCH
+        if (mm) hipLaunchKernelGGL(k_node_batch<true>, gn, dim3(NODE_BLOCK), 0, s, g[EB_NODE]);
+        else hipLaunchKernelGGL(k_node_batch<false>, gn, dim3(NODE_BLOCK), 0, s, g[EB_NODE]);
+    }
+    hipLaunchKernelGGL(k_geo_batch, gg, dim3(GEO_BLOCK), 0, s, g[EB_GEO]);
+    hipLaunchKernelGGL(k_fin_batch, dim3(grp.idx.size()), dim3(FIN_BLOCK), 0, s, grp.base[q ^ 1],
+                       (const BatchMember*)grp.dmem[q ^ 1], (const int*)grp.dcount[EB_GEO],
+                       (const int*)grp.dcount[EB_NODE]);
+    HIPCHK(hipGetLastError());
+    return SNH_OK;
 }
 
 extern "C" int snh_ensemble_create(snh_context* const* members, int32_t n, int32_t graph_steps,
@@ -3996,6 +4436,12 @@ extern "C" int snh_ensemble_create(snh_context* const* members, int32_t n, int32
         for (snh_context* c : e->m)
             if (hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(SNH_EHIP, "member sync failed");
         if (rc) break;
+        e->batched = ens_batchable(e->m);
+        if (const char* env = getenv("SNH_ENSEMBLE_TW")) e->tw = env[0] != '0';
+        if (e->batched && (rc = ens_make_groups(e))) break;
+        for (size_t k = 0; e->batched && k < e->groups.size() && rc == SNH_OK; k++)
+            rc = ens_batch_tables(e, e->groups[k]);
+        if (rc) break;
         /* capture records the launches without running them; the host-side
            buffer index is set for each of the two parities and restored */
         for (int par = 0; par < 2 && rc == SNH_OK; par++) {
@@ -4007,7 +4453,18 @@ extern "C" int snh_ensemble_create(snh_context* const* members, int32_t n, int32
             int crc = SNH_OK;
             if (hipEventRecord(e->fork, e->origin) != hipSuccess) crc = SNH_EHIP;
             std::vector<uint32_t> t0(n);
-            for (int i = 0; i < n && crc == SNH_OK; i++) {
+            for (int i = 0; i < n; i++) t0[i] = e->m[i]->timeStep;
+            /* batched: each group's steps on its own stream, forked from and
+               joined to the origin */
+            for (size_t k = 0; e->batched && k < e->groups.size() && crc == SNH_OK; k++) {
+                const EnsGroup& grp = e->groups[k];
+                if (hipStreamWaitEvent(grp.stream, e->fork, 0) != hipSuccess) crc = SNH_EHIP;
+                for (int s = 0; s < graph_steps && crc == SNH_OK; s++)
+                    crc = ens_batch_step(e, grp, grp.stream, par ^ (s & 1));
+                if (crc == SNH_OK && hipEventRecord(grp.join, grp.stream) != hipSuccess) crc = SNH_EHIP;
+                if (crc == SNH_OK && hipStreamWaitEvent(e->origin, grp.join, 0) != hipSuccess) crc = SNH_EHIP;
+            }
+            for (int i = 0; !e->batched && i < n && crc == SNH_OK; i++) {
                 snh_context* c = e->m[i];
                 t0[i] = c->timeStep;
                 if (hipStreamWaitEvent(c->stream, e->fork, 0) != hipSuccess) crc = SNH_EHIP;
@@ -4076,6 +4533,11 @@ extern "C" int snh_ensemble_step(snh_ensemble* e, int32_t nsteps) {
         for (snh_context* c : e->m) HIPCHK(hipStreamWaitEvent(c->stream, e->fork, 0));
     }
     return SNH_OK;
+}
+
+extern "C" int snh_ensemble_batched(const snh_ensemble* e) {
+    if (!e) return set_err(SNH_EINVAL, "null ensemble");
+    return e->batched ? 1 : 0;
 }
 
 extern "C" int snh_ensemble_synchronize(snh_ensemble* e) {

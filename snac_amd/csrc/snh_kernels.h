@@ -160,12 +160,14 @@ SNH_HD void face_cross(const double* c1, const double* c2, const double* c3, dou
 
 /* Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD
    b % 8), each with its own L2.  Renumber so that every XCD works through one
-   contiguous range of the mesh: elements sharing nodes then share an L2. */
-__device__ __forceinline__ int xcd_block() {
-    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+   contiguous range of the mesh: elements sharing nodes then share an L2.
+   (b, nb): a block of a range of nb blocks that starts on a multiple of 8
+   (a member of a batched ensemble launch). */
+__device__ __forceinline__ int xcd_block(const int b, const int nb) {
     const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
     return x * q + (x < r ? x : r) + i;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_block((int)blockIdx.x, (int)gridDim.x); }
 
 /* Tetrahedra_Volume (Tetrahedra.c:44-67).  Its cross product is the one of
    face 0 (nodes 0,1,2): x3*y2 == y2*x3 exactly. */

@@ -83,7 +83,7 @@ __device__ __forceinline__ void rotate_stress_mx(double st[6], const float w[3],
 
 /* TW as in k_elem: one wave per tet index over 64 elements */
 template <int RSET, bool TW>
-__global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
+__device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const int nblk) {
     constexpr int NE = TW ? 64 : EPW;
     constexpr int NT = TW ? NTET + 1 : NTET;
     __shared__ double sh_a[NE][NT];
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
     const bool in_wave = TW || lane < EPW * NTET;
     const int le = TW ? (lane & 63) : (in_wave ? lane / NTET : 0);
     const int t = TW ? (lane >> 6) : (in_wave ? lane - le * NTET : 0);
-    const int e = xcd_block() * NE + le;
+    const int e = xcd_block(blk, nblk) * NE + le;
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
@@ -299,10 +299,15 @@ __global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
     }
 }
 
+template <int RSET, bool TW>
+__global__ void __launch_bounds__(TW ? 640 : 64) k_elem_mx(StepArgs a) {
+    elem_pass_mx<RSET, TW>(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
 /* node gather, mixed precision */
-__global__ void __launch_bounds__(NODE_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) k_node_mx(StepArgs a) {
+__device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const int nblk) {
     __shared__ double sh[NODE_BLOCK / 64];
-    const int g = xcd_block() * blockDim.x + threadIdx.x;
+    const int g = xcd_block(blk, nblk) * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
     if (g < a.N) {
         /* the FP64 gather's banded node order: node rows that read one
@@ -377,7 +382,11 @@ __global__ void __launch_bounds__(NODE_BLOCK) __attribute__((amdgpu_waves_per_eu
         vmag = node_finish(a, n, f0, f1, f2, im, xn0, xn1, xn2, dt);
     }
     double bm = block_max(vmag, sh);
-    if (threadIdx.x == 0) a.part_vmax[blockIdx.x] = bm;
+    if (threadIdx.x == 0) a.part_vmax[blk] = bm;
+}
+
+__global__ void __launch_bounds__(NODE_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) k_node_mx(StepArgs a) {
+    node_pass_mx(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
 #endif

@@ -1,8 +1,9 @@
 """Ensembles: several independent models stepped together by one HIP graph
 (snh_ensemble_* in the C ABI).  Each member keeps its own context, stream,
-materials, BCs and element-pass variant (chosen per member by the
-autotuner); a graph launch advances all of them graph_steps steps, members on
-parallel branches.
+materials and BCs; a graph launch advances all of them graph_steps steps.
+Members of one precision and one law are stepped by batched launches (one
+launch of each pass over all members' blocks per step); others by parallel
+per-member branches, each member in its autotuned element-pass variant.
 
     members = [Context(p).Construct().Build().Initialise() for p in problems]
     ens = Ensemble(members, graph_steps=10, autotune=True)
@@ -36,6 +37,14 @@ class Ensemble:
         h = C.c_void_p()
         _abi.check(self.lib.snh_ensemble_create(arr, len(members), int(graph_steps), C.byref(h)), self.lib)
         self.h = h
+
+    @property
+    def batched(self) -> bool:
+        """True when one batched launch per pass steps all members."""
+        rc = self.lib.snh_ensemble_batched(self.h)
+        if rc < 0:
+            _abi.check(rc, self.lib)
+        return rc == 1
 
     def step(self, nsteps: int):
         _abi.check(self.lib.snh_ensemble_step(self.h, int(nsteps)), self.lib)

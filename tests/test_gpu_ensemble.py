@@ -1,6 +1,7 @@
-"""Ensembles (snh_ensemble_*): members stepped by one HIP graph give the
-same bits as each member stepped alone; contexts alive together keep their
-own material and BC tables."""
+"""Ensembles (snh_ensemble_*): members stepped by one HIP graph, as
+per-member branches or as batched launches over all members, give the same
+bits as each member stepped alone; contexts alive together keep their own
+material and BC tables."""
 import numpy as np
 import pytest
 
@@ -73,6 +74,66 @@ def test_mixed_precision_ensemble_matches_members_alone():
         m.Destroy()
     for p, got in zip(probs, states):
         ref = _alone(p, 24)
+        for f in FIELDS:
+            assert np.array_equal(ref[f], got[f]), (p.name, f)
+
+
+def _two_elastic(p):
+    """The half i >= EI/2 another elastic material: one law, a material map."""
+    soft = P.MaterialSpec(rheology=_abi.RHEOLOGY_ELASTIC, alpha=0.0, beta=0.0, lam=1.0e10, mu=1.5e10,
+                          density=2700.0)
+    p.materials = [p.materials[0], soft]
+    ni, nj, nk = p.elements
+    mat = np.zeros((nk, nj, ni), dtype=np.uint8)
+    mat[:, :, ni // 2:] = 1
+    p.element_material = mat.reshape(-1)
+    return p
+
+
+BATCH_CASES = {
+    "mc_fp64": lambda: elastoplastic_ensemble(n_models=5, nx=(6, 14), ny=(3, 6), seed=4),
+    "mc_mixed": lambda: [_mixed(p) for p in elastoplastic_ensemble(n_models=5, nx=(6, 14), ny=(3, 6), seed=4)],
+    "maxwell_mixed": lambda: [_mixed(P.maxwell_viscoelastic(elements=e)) for e in [(5, 3, 5), (9, 4, 7), (4, 5, 6)]],
+    "elastic_materials": lambda: [_two_elastic(P.elastic_box_compression(elements=e))
+                                  for e in [(6, 3, 6), (10, 4, 8), (7, 5, 4)]],
+}
+
+
+@pytest.mark.parametrize("case", list(BATCH_CASES))
+def test_batched_ensemble_matches_members_alone(case):
+    """Members of one precision and law are stepped by one launch per pass
+    over all their blocks; each gives the bits it gives alone, through graph
+    launches and the remaining plain steps."""
+    probs = BATCH_CASES[case]()
+    members = [Context(p).Construct().Build().Initialise() for p in probs]
+    ens = Ensemble(members, graph_steps=4, autotune=True)
+    assert ens.batched
+    for n in (8, 3, 9):
+        ens.step(n)
+        ens.synchronize()
+    states = [m.state(fields=FIELDS) for m in members]
+    ens.destroy()
+    for m in members:
+        m.Destroy()
+    for p, got in zip(probs, states):
+        ref = _alone(p, 20)
+        for f in FIELDS:
+            assert np.array_equal(ref[f], got[f]), (case, p.name, f)
+
+
+def test_batching_can_be_switched_off(monkeypatch):
+    probs = elastoplastic_ensemble(n_models=3, nx=(6, 10), ny=(3, 5), seed=6)
+    monkeypatch.setenv("SNH_ENSEMBLE_BATCH", "0")
+    members = [Context(p).Construct().Build().Initialise() for p in probs]
+    ens = Ensemble(members, graph_steps=4)
+    assert not ens.batched
+    ens.step(8).synchronize()
+    states = [m.state(fields=FIELDS) for m in members]
+    ens.destroy()
+    for m in members:
+        m.Destroy()
+    for p, got in zip(probs, states):
+        ref = _alone(p, 8)
         for f in FIELDS:
             assert np.array_equal(ref[f], got[f]), (p.name, f)
 
