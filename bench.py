@@ -432,12 +432,17 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs behind roofline.traffic")
     ap.add_argument("--no-mixed-c3", action="store_true", help="skip the mixed-precision Maxwell line")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the 64-model ensemble line (for profiling it), printed as JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
               "(sharding over several GPUs is designed, not built).", file=sys.stderr)
         sys.exit(2)
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(64, args.ensemble_steps)}))
+        return
 
     # counter runs first, in child processes, before this process touches the GPU
     traffic = None if args.no_traffic else hbm_traffic(args)
