@@ -463,14 +463,16 @@ int snh_adjoint_end(snh_context* ctx);
  *
  * Batched launches.  When the members share one precision and one single
  * law (elastic, Mohr-Coulomb or Maxwell), the same material layout, the
- * per-node node pass and no dump diagnostics or tet densities, the graph
- * instead holds, per step, one launch of each pass over all members' blocks
- * (the element pass in the tet-per-lane layout whatever the members'
- * autotuned variants, which give the same bits): 4 launches per step for
- * the whole ensemble instead of 4 per member.  Environment at creation:
- * SNH_ENSEMBLE_BATCH=0 keeps the per-member branches; SNH_ENSEMBLE_TW=1
- * takes the tet-per-wave layout; SNH_ENSEMBLE_GROUPS=k splits the members
- * into k groups of about equal size, batched on k streams. */
+ * per-node node pass and no dump diagnostics or tet densities, and average
+ * fewer than 16384 elements (where launches, not kernels, set the time),
+ * the graph instead holds, per step, one launch of each pass over all
+ * members' blocks (the element pass in the tet-per-lane layout whatever the
+ * members' autotuned variants, which give the same bits): 4 launches per
+ * step for the whole ensemble instead of 4 per member.  Environment at
+ * creation: SNH_ENSEMBLE_BATCH=1 batches whatever the sizes, =0 never;
+ * SNH_ENSEMBLE_TW=1 takes the tet-per-wave layout; SNH_ENSEMBLE_GROUPS=k
+ * splits the members into k groups of about equal size, batched on k
+ * streams. */
 typedef struct snh_ensemble snh_ensemble;
 int snh_ensemble_create(snh_context* const* members, int32_t n, int32_t graph_steps, snh_ensemble** out);
 /* 1 when the ensemble steps its members by batched launches, 0 when by

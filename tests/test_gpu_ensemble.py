@@ -138,6 +138,31 @@ def test_batching_can_be_switched_off(monkeypatch):
             assert np.array_equal(ref[f], got[f]), (p.name, f)
 
 
+def test_large_members_take_branches_unless_batching_is_forced(monkeypatch):
+    """Members averaging 16384 elements or more are stepped as per-member
+    branches by default (there the kernels, not the launches, set the time);
+    SNH_ENSEMBLE_BATCH=1 batches them.  The same bits either way."""
+    probs = [P.mohr_coulomb_extension(elements=(32, 16, 32), seed=2),
+             P.mohr_coulomb_extension(elements=(40, 16, 30), seed=3)]
+    runs = {}
+    for force in (None, "1"):
+        if force:
+            monkeypatch.setenv("SNH_ENSEMBLE_BATCH", force)
+        members = [Context(p).Construct().Build().Initialise() for p in probs]
+        ens = Ensemble(members, graph_steps=4, autotune=False)
+        runs[force] = ens.batched
+        ens.step(8).synchronize()
+        states = [m.state(fields=FIELDS) for m in members]
+        ens.destroy()
+        for m in members:
+            m.Destroy()
+        for p, got in zip(probs, states):
+            ref = _alone(p, 8)
+            for f in FIELDS:
+                assert np.array_equal(ref[f], got[f]), (force, p.name, f)
+    assert runs == {None: False, "1": True}
+
+
 def test_live_contexts_keep_their_own_tables():
     """Two contexts with different materials and BCs, stepped alternately."""
     pa = P.elastic_box_compression(elements=(6, 3, 6))
