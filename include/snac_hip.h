@@ -222,7 +222,9 @@ int snh_hydrostatic_ic(snh_context* ctx);
  * previous coordinates (the old_volume of Maxwell's next volumetric
  * strain), the new mesh's Jaumann rotation with the dt of the last Sync is
  * applied at the next CalcStresses, and minLengthScale becomes the minimum
- * of its current value and the new mesh's; dt is not recomputed.  FP64. */
+ * of its current value and the new mesh's; dt is not recomputed.  FP64 and
+ * mixed-precision contexts (both hold the state in FP64; a pending rotation
+ * is applied by the FP64 kernel). */
 int snh_remesh(snh_context* ctx, const snh_state* remeshed);
 
 /* One condition of the temperatureBCs composite (a composite of wall

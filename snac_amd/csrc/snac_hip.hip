@@ -3393,7 +3393,8 @@ extern "C" int snh_remesh(snh_context* c, const snh_state* s) {
     if (!c || !s) return set_err(SNH_EINVAL, "null argument");
     if (!c->zero_done) return set_err(SNH_ESTATE, "snh_remesh before snh_timestep_zero");
     if (c->adj) return set_err(SNH_ESTATE, "snh_remesh during an adjoint recording");
-    if (c->cfg.precision != SNH_PRECISION_FP64) return set_err(SNH_EUNSUPPORTED, "snh_remesh: FP64 path only");
+    /* a mixed-precision context holds the same FP64 state; its pending
+       rotation is applied here by the FP64 kernel */
     int rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     const size_t N = c->N, E = c->E;

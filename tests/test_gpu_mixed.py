@@ -142,3 +142,33 @@ def test_recorded_schedule_replays_the_run_bit_for_bit(prec):
     b.Destroy()
     for f in ("coord", "velocity", "stress", "plastic_strain"):
         assert np.array_equal(sa[f], sb[f]), f
+
+
+def test_mixed_remesh_follows_fp64():
+    """snh_remesh on a mixed-precision context (its FP64 state takes the
+    remeshed arrays; the pending rotation is applied in FP64): two remesh
+    handoffs inside an undamped Maxwell run, against the FP64 context with
+    the same handoffs, topography within 1e-6."""
+    from tests.test_gpu_remesh import _host_remesh
+    runs = {}
+    for prec in (_abi.PRECISION_FP64, _abi.PRECISION_MIXED):
+        p = P.maxwell_viscoelastic(elements=(8, 5, 7))
+        p.demf = 0.0
+        p.precision = prec
+        c = Context(p).Construct().Build().Initialise()
+        x_init = c.state(fields=("coord",))["coord"]
+        for _ in range(2):
+            c.Execute(15)
+            s = c.state(fields=("coord", "velocity", "stress"))
+            c.remesh(**_host_remesh(p, s, x_init))
+        c.Execute(15)
+        runs[prec] = (x_init, c.state(fields=("coord", "stress")))
+        c.Destroy()
+    x0, a = runs[_abi.PRECISION_FP64]
+    _, b = runs[_abi.PRECISION_MIXED]
+    I, J, K = P.maxwell_viscoelastic(elements=(8, 5, 7)).nodes
+    top = (np.arange(I * J * K) // I) % J == J - 1
+    ta, tb = a["coord"][top, 1] - x0[top, 1], b["coord"][top, 1] - x0[top, 1]
+    err = float(np.max(np.abs(tb - ta)) / np.max(np.abs(ta)))
+    print(f"  mixed vs fp64 with two remeshes, undamped: topography {err:.2e}", flush=True)
+    assert np.all(np.isfinite(b["stress"])) and err <= 1e-6
