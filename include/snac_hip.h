@@ -347,8 +347,11 @@ int snh_set_node_variant(snh_context* ctx, int32_t variant);
  * element strain-rate invariant (StrainRate.c:272-304), stress invariant and
  * hydrostatic pressure (Stress.c:140-156), the volume-averaged plastic strain
  * (plastic_BI/Constitutive.c:232-236) and the damped nodal force
- * (UpdateNode.c:102-104).  Enable it for the steps that end on a dump
- * (FP64 contexts only; SNH_EUNSUPPORTED for mixed precision). */
+ * (UpdateNode.c:102-104).  Enable it for the steps that end on a dump.
+ * On a mixed-precision context the element quantities come from the FP64
+ * element pass run on the same state into scratch before the mixed pass
+ * (one more element pass per recorded step), the force from the mixed node
+ * pass itself. */
 typedef struct snh_diagnostics {
     double* strain_rate;      /* [E] or NULL */
     double* stress;           /* [E] or NULL */

@@ -2538,6 +2538,8 @@ struct snh_context {
     bool diag_on;             /* steps record dump diagnostics */
     double* diag_buf;         /* [5][E] */
     double* force_buf;        /* [3][N] */
+    double* diag_mx;          /* mixed precision: the FP64 diagnostic pass's
+                                 stress [60E] and plastic strain [10E] */
     /* SnacTemperature (snh_set_thermal) */
     bool thermal;
     bool thermal_stress;
@@ -2730,7 +2732,7 @@ static double wave_speed_max(double cur, const snh_material& m) {
 static void free_all(snh_context* c) {
     void* ptrs[] = {c->x[0], c->x[1], c->v, c->stress, c->ps, c->visc, c->mat, c->refvisc, c->cohoff,
                     c->bc, c->part_min, c->part_vmax, c->scal, c->tmp, c->sched_dt, c->sched_c, c->sched_damp,
-                    c->dmat, c->dbc, c->diag_buf, c->force_buf, c->temp, c->temp0, c->tpre, c->hflux,
+                    c->dmat, c->dbc, c->diag_buf, c->force_buf, c->diag_mx, c->temp, c->temp0, c->tpre, c->hflux,
                     c->tstress, c->dens, c->tbc, c->tbcv, c->stage, c->topo_rec, c->topo_pmax,
                     c->topo_pfirst, c->rec_code};
     for (void* p : ptrs)
@@ -2963,6 +2965,7 @@ extern "C" int snh_context_create(const snh_config* cfg, snh_context** out) {
     c->adj = nullptr;
     c->diag_on = false;
     c->diag_buf = c->force_buf = nullptr;
+    c->diag_mx = nullptr;
     c->sched_dt = c->sched_c = nullptr;
     c->sched_damp = nullptr;
     c->sched_base = 0;
@@ -3135,8 +3138,20 @@ static int launch_fin(snh_context* c) {
 }
 
 static int launch_elem_args(snh_context* c, StepArgs a);
+static int launch_elem_fp64(snh_context* c, StepArgs a);
 static int launch_elem(snh_context* c) {
     int rc;
+    if (c->cfg.precision == SNH_PRECISION_MIXED && c->diag_on) {
+        /* the dump's element quantities of a mixed-precision step: the FP64
+           element pass on the same state, its stress and plastic strain
+           written to scratch, before the mixed pass updates the state */
+        StepArgs d = make_args(c);
+        d.stress_out = c->diag_mx;
+        d.ps_out = c->ps ? c->diag_mx + 60 * (size_t)c->E : nullptr;
+        d.visc = nullptr;
+        d.force_out = nullptr;
+        if ((rc = launch_elem_fp64(c, d))) return rc;
+    }
     {
         KTimer kt(c, KK_ELEM);
         rc = launch_elem_args(c, make_args(c));
@@ -3153,9 +3168,10 @@ static int launch_elem(snh_context* c) {
     }
     return rc;
 }
+static int launch_elem_fp64(snh_context* c, StepArgs a);
+
 static int launch_elem_args(snh_context* c, StepArgs a) {
     dim3 grid((c->E + EPW - 1) / EPW), block(64);
-    const bool mm = a.mat != nullptr;
     if (c->cfg.precision == SNH_PRECISION_MIXED) {
         /* tet per wave unless the tet-per-lane layout is forced */
         const bool tw = c->elem_variant != SNH_ELEM_TET_PER_LANE;
@@ -3178,6 +3194,13 @@ static int launch_elem_args(snh_context* c, StepArgs a) {
         HIPCHK(hipGetLastError());
         return SNH_OK;
     }
+    return launch_elem_fp64(c, a);
+}
+
+/* the FP64 element pass in the context's layout */
+static int launch_elem_fp64(snh_context* c, StepArgs a) {
+    dim3 grid((c->E + EPW - 1) / EPW), block(64);
+    const bool mm = a.mat != nullptr;
     /* the diagnostics are written by the element-per-lane variant only */
     if (a.diag || c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE ||
         c->elem_variant == SNH_ELEM_ELEMENT_PER_LANE_LOWREG) {
@@ -3579,8 +3602,8 @@ extern "C" int snh_step(snh_context* c, int nsteps) {
 
 extern "C" int snh_set_diagnostics(snh_context* c, int enable) {
     if (!c) return set_err(SNH_EINVAL, "null context");
-    if (enable && c->cfg.precision != SNH_PRECISION_FP64)
-        return set_err(SNH_EUNSUPPORTED, "dump diagnostics are written by the FP64 path");
+    if (enable && c->cfg.precision == SNH_PRECISION_MIXED && !c->diag_mx)
+        HIPCHK(hipMalloc((void**)&c->diag_mx, 70 * (size_t)c->E * sizeof(double)));
     if (enable && !c->diag_buf) {
         HIPCHK(hipMalloc((void**)&c->diag_buf, 5 * (size_t)c->E * sizeof(double)));
         HIPCHK(hipMalloc((void**)&c->force_buf, 3 * (size_t)c->N * sizeof(double)));

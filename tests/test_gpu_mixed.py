@@ -112,15 +112,30 @@ def test_mixed_layouts_agree():
         assert np.array_equal(a[f], b[f]), f
 
 
-def test_mixed_rejects_diagnostics():
-    p = P.elastic_box_compression(elements=(4, 3, 4))
-    p.precision = _abi.PRECISION_MIXED
-    c = Context(p).Construct().Build().Initialise()
-    try:
-        with pytest.raises(Exception):
-            c.set_diagnostics(True)
-    finally:
+@pytest.mark.parametrize("make", [lambda: P.mohr_coulomb_shear(elements=(6, 4, 6)),
+                                  lambda: P.maxwell_viscoelastic(elements=(6, 4, 6))], ids=["mc_shear", "maxwell"])
+def test_mixed_diagnostics_follow_fp64(make):
+    """Dump diagnostics of a mixed-precision step (element quantities from the
+    FP64 element pass on the mixed state, force from the mixed node pass)
+    against those of the FP64 run, undamped, after 20 steps."""
+    out = {}
+    for prec in (_abi.PRECISION_FP64, _abi.PRECISION_MIXED):
+        p = make()
+        p.demf = 0.0
+        p.precision = prec
+        c = Context(p).Construct().Build().Initialise()
+        c.Execute(19)
+        c.set_diagnostics(True)
+        c.Execute(1)
+        out[prec] = c.diagnostics()
         c.Destroy()
+    a, b = out[_abi.PRECISION_FP64], out[_abi.PRECISION_MIXED]
+    for k in ("strain_rate", "stress", "hydro_pressure", "plastic_strain", "force"):
+        scale = float(np.max(np.abs(a[k])))
+        err = float(np.max(np.abs(b[k] - a[k]))) / scale if scale > 0 else float(np.max(np.abs(b[k])))
+        print(f"    {k}: mixed vs fp64 {err:.2e} (scale {scale:.3e})", flush=True)
+        assert np.all(np.isfinite(b[k]))
+        assert err <= (1e-4 if k == "force" else 1e-5), k
 
 
 @pytest.mark.parametrize("prec", [_abi.PRECISION_FP64, _abi.PRECISION_MIXED], ids=["fp64", "mixed"])
