@@ -249,8 +249,8 @@ typedef struct snh_temperature_bc {
  * computeThermalStress, the stress of those laws.  Call after create and
  * before snh_timestep_zero; temperature[N] is the initial nodal field (the
  * nodeICs), to which _SnacTemperature_InitialConditions (VariableConditions.c:
- * 155-212) applies the BCs.  FP64 contexts only (SNH_EUNSUPPORTED for
- * mixed precision and the adjoint). */
+ * 155-212) applies the BCs.  FP64 and mixed-precision contexts (the heat
+ * passes run in FP64 in both); not with the adjoint. */
 typedef struct snh_thermal {
     int32_t  computeThermalStress;   /* Context.c:756-757                   */
     uint32_t nbcs;

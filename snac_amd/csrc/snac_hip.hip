@@ -3354,8 +3354,6 @@ static int fill_tet_density(snh_context* c) {
 extern "C" int snh_set_thermal(snh_context* c, const snh_thermal* th) {
     if (!c || !th) return set_err(SNH_EINVAL, "null argument");
     if (th->nbcs && !th->bcs) return set_err(SNH_EINVAL, "null temperature BC list");
-    if (c->cfg.precision != SNH_PRECISION_FP64)
-        return set_err(SNH_EUNSUPPORTED, "the temperature plugin runs on the FP64 path");
     if (c->zero_done) return set_err(SNH_ESTATE, "snh_set_thermal after snh_timestep_zero");
     if (c->adj) return set_err(SNH_ESTATE, "snh_set_thermal during an adjoint recording");
     for (uint32_t k = 0; k < c->cfg.nmaterials; k++) {

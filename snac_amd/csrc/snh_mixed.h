@@ -247,13 +247,14 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
         const double v_old = tet_volume(Xp[0], Xp[1], Xp[2], Xp[3]);
         /* a difference of nearly equal FP64 volumes, then an FP32 quotient */
         const double trace_strain = (double)((float)(v_new - v_old) / (float)v_old);
+        add_thermal_stress(a, st, t, ec);
         const double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
         double visc;
         {
             const double te3 = trace_strain * (1.0 / 3.0);
             double d0 = sn[0] - te3, d1 = sn[1] - te3, d2 = sn[2] - te3;
             const double snd[6] = {sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]};
-            visc = viscosity_of(m, rvisc, d0, d1, d2, snd, dt);
+            visc = viscosity_of(m, rvisc, d0, d1, d2, snd, dt, tet_temp(a, m, nd));
         }
         if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
         /* sigma'_new = sigma' + sigma'(-2x/(1+x)) + 2 mu e'/(1+x), x = mu dt/(2 eta) */
@@ -296,6 +297,7 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
     if (active) {
 #pragma unroll
         for (int c = 0; c < 6; c++) a.stress_out[((size_t)c * NTET + t) * E + ec] = st[c];
+        if (a.tpre) store_density(a, m, nd, t, ec);
     }
 }
 
@@ -336,6 +338,7 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
             for (int q = 0; q < 6; q++) sref[q] = a.stress[((size_t)q * NTET + tref) * E + e];
             double C0 = 0.0, C1 = 0.0, C2 = 0.0;   /* sum of FP64 face vectors (2 n A) */
             double vs = 0.0;                         /* sum of the 5 tet volumes, FP64 */
+            double vr = 0.0;                         /* with tet densities: sum of rho V */
             float g0 = 0.f, g1 = 0.f, g2 = 0.f;
 #pragma unroll
             for (int idx = 0; idx < 5; idx++) {
@@ -356,8 +359,10 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
                    FP64: the gravity load it scales is balanced by the
                    FP64 pre-stress term, so its rounding error is not
                    damped by the force balance. */
-                vs += fabs((X[qo][0] - X[q0][0]) * dax + (X[qo][1] - X[q0][1]) * day +
-                           (X[qo][2] - X[q0][2]) * daz);
+                const double vt = fabs((X[qo][0] - X[q0][0]) * dax + (X[qo][1] - X[q0][1]) * day +
+                                       (X[qo][2] - X[q0][2]) * daz);
+                vs += vt;
+                if (a.dens) vr += a.dens[(size_t)t * E + e] * vt;
                 const float ax = (float)dax, ay = (float)day, az = (float)daz;   /* 2 n A */
                 float ds[6];
 #pragma unroll
@@ -367,7 +372,9 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
                 g2 += ds[4] * ax + ds[5] * ay + ds[2] * az;
             }
             vs *= 1.0 / 6.0;
-            const double gr = a.gravity * m.rho_eff * 0.125 * vs;
+            /* the tet densities of the temperature plugin's last Stress call,
+               or the material's */
+            const double gr = a.gravity * 0.125 * (a.dens ? vr * (1.0 / 6.0) : m.rho_eff * vs);
             const double mi = 0.25 * (m.lam2mu * rc2) * vs;
             /* sigma . n A / 6 = sigma . (2 n A) / 12 */
             constexpr double r12 = 1.0 / 12.0;

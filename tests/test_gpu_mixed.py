@@ -187,3 +187,32 @@ def test_mixed_remesh_follows_fp64():
     err = float(np.max(np.abs(tb - ta)) / np.max(np.abs(ta)))
     print(f"  mixed vs fp64 with two remeshes, undamped: topography {err:.2e}", flush=True)
     assert np.all(np.isfinite(b["stress"])) and err <= 1e-6
+
+
+@pytest.mark.parametrize("activationE", [0.0, 5.4e3], ids=["linear", "arrhenius"])
+def test_mixed_temperature_plugin_follows_fp64(activationE):
+    """The temperature plugin on a mixed-precision context: the heat passes
+    run in FP64 as on the FP64 path, and the mixed element and node passes
+    take the temperature through the Arrhenius viscosity, the thermal stress
+    and the tet densities.  Undamped, 40 steps, against the FP64 run."""
+    out = {}
+    for prec in (_abi.PRECISION_FP64, _abi.PRECISION_MIXED):
+        p = P.maxwell_thermal(elements=(8, 5, 7), activationE=activationE)
+        p.demf = 0.0
+        p.precision = prec
+        c = Context(p).Construct().Build().Initialise()
+        x0 = c.state(fields=("coord",))["coord"]
+        c.Execute(40)
+        out[prec] = (x0, c.state(fields=("coord", "stress", "temperature")))
+        c.Destroy()
+    x0, a = out[_abi.PRECISION_FP64]
+    _, b = out[_abi.PRECISION_MIXED]
+    I, J, K = P.maxwell_thermal(elements=(8, 5, 7)).nodes
+    top = (np.arange(I * J * K) // I) % J == J - 1
+    ta, tb = a["coord"][top, 1] - x0[top, 1], b["coord"][top, 1] - x0[top, 1]
+    rel = lambda x, y: float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+    errs = {"topography": rel(tb, ta), "stress": rel(b["stress"], a["stress"]),
+            "temperature": rel(b["temperature"], a["temperature"])}
+    print("    mixed vs fp64 with the temperature plugin: "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert errs["topography"] <= 1e-6 and errs["temperature"] <= 1e-9 and errs["stress"] <= 1e-6
