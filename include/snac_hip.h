@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SNH_ABI_VERSION 5
+#define SNH_ABI_VERSION 6
 
 /* ---- status codes ------------------------------------------------------- */
 #define SNH_OK            0
@@ -319,10 +319,20 @@ int snh_profile(snh_context* ctx, int enable);
 #define SNH_ELEM_ELEMENT_PER_LANE_LOWREG 2   /* kinematics recomputed, not kept */
 #define SNH_ELEM_TET_PER_WAVE            3   /* 64 elements per block, one wave per tet */
 #define SNH_ELEM_VARIANTS                4
+#define SNH_NODE_TUNES 4   /* node pass: per node; tiled with 4, 8, 16 planes */
 typedef struct snh_tune_report {
     double   elem_ms[SNH_ELEM_VARIANTS];  /* best launch time per variant, ms */
     int32_t  elem_variant;                /* chosen */
+    double   node_ms[SNH_NODE_TUNES];     /* FP64 node pass: per-node gather, tiled
+                                             with 4, 8 and 16 node planes per
+                                             workgroup; 0 when not timed */
+    int32_t  node_variant;                /* chosen SNH_NODE_* */
+    int32_t  node_kc;                     /* and its node planes per workgroup */
 } snh_tune_report;
+/* Time the element-pass layouts and (FP64 contexts) the node-pass variants
+ * and tile heights on the context's own mesh, law and state, outputs to
+ * scratch, and keep the fastest of each; every choice gives the same bits.
+ * SNH_ELEM_VARIANT / SNH_NODE_VARIANT in the environment pin the choice. */
 int snh_autotune(snh_context* ctx, snh_tune_report* out);
 /* Force one element-pass variant (SNH_ELEM_*); both give identical results. */
 int snh_set_elem_variant(snh_context* ctx, int32_t variant);

@@ -149,8 +149,12 @@ NODE_PER_NODE = 0      # SNH_NODE_* of include/snac_hip.h
 NODE_TILED = 1
 
 
+NODE_TUNES = 4          # SNH_NODE_TUNES
+
+
 class TuneReport(C.Structure):
-    _fields_ = [("elem_ms", C.c_double * ELEM_VARIANTS), ("elem_variant", C.c_int32)]
+    _fields_ = [("elem_ms", C.c_double * ELEM_VARIANTS), ("elem_variant", C.c_int32),
+                ("node_ms", C.c_double * NODE_TUNES), ("node_variant", C.c_int32), ("node_kc", C.c_int32)]
 
 
 class Diagnostics(C.Structure):
@@ -227,7 +231,7 @@ SIGNATURES = {
     "snh_get_kernel_times": (_I, [_P, C.POINTER(KernelTimes)]),
 }
 
-ABI_VERSION = 5          # SNH_ABI_VERSION of include/snac_hip.h
+ABI_VERSION = 6          # SNH_ABI_VERSION of include/snac_hip.h
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libsnac_hip.so")

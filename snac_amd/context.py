@@ -123,12 +123,16 @@ class Context:
         self._check(self.lib.snh_synchronize(self.h))
 
     def autotune(self) -> Dict[str, float]:
-        """Pick the fastest element-pass variant for this mesh and law."""
+        """Pick the fastest element-pass layout and (FP64) node-pass variant
+        and tile height for this mesh, law and state."""
         r = _abi.TuneReport()
         self._check(self.lib.snh_autotune(self.h, C.byref(r)))
         return {"elem_ms_tet_per_lane": r.elem_ms[0], "elem_ms_element_per_lane": r.elem_ms[1],
                 "elem_ms_element_per_lane_lowreg": r.elem_ms[2], "elem_ms_tet_per_wave": r.elem_ms[3],
-                "elem_variant": r.elem_variant}
+                "elem_variant": r.elem_variant,
+                "node_ms_per_node": r.node_ms[0], "node_ms_tiled_kc4": r.node_ms[1],
+                "node_ms_tiled_kc8": r.node_ms[2], "node_ms_tiled_kc16": r.node_ms[3],
+                "node_variant": r.node_variant, "node_kc": r.node_kc}
 
     def set_element_material(self, material):
         """Per-element material index, [E] (snh_set_element_material)."""

@@ -3922,6 +3922,79 @@ extern "C" int snh_set_node_variant(snh_context* c, int32_t v) {
     return SNH_OK;
 }
 
+static int tile_blocks(const snh_context* c, int kc) {
+    return ((c->I + NT_I - 1) / NT_I) * ((c->J + NT_J - 1) / NT_J) * ((c->K + kc - 1) / kc);
+}
+
+/* the FP64 node pass: the per-node gather and the tiled pass with 4, 8 and
+   16 node planes per workgroup, each timed on the context's own state with
+   its velocity, coordinate and partial-maximum outputs in scratch; keeps the
+   fastest (all give the same bits) */
+static int autotune_node(snh_context* c, snh_tune_report& rep) {
+    static const int kcs[SNH_NODE_TUNES - 1] = {4, 8, 16};
+    const size_t N = c->N;
+    int maxb = c->n_node_blocks;
+    for (int kc : kcs) maxb = std::max(maxb, tile_blocks(c, kc));
+    double *sv = nullptr, *sx = nullptr, *spv = nullptr;
+    bool ok = hipMalloc((void**)&sv, 3 * N * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&sx, 3 * N * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&spv, (size_t)maxb * sizeof(double)) == hipSuccess;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    int rc = ok ? SNH_OK : set_err(SNH_ENOMEM, "autotune scratch allocation failed");
+    const int saved_v = c->node_variant, saved_kc = c->node_kc, saved_tb = c->n_tile_blocks;
+    for (int o = 0; o < SNH_NODE_TUNES && rc == SNH_OK; o++) {
+        c->node_variant = o == 0 ? SNH_NODE_PER_NODE : SNH_NODE_TILED;
+        c->node_kc = o == 0 ? saved_kc : kcs[o - 1];
+        c->n_tile_blocks = tile_blocks(c, c->node_kc);
+        StepArgs a = make_args(c);
+        a.v_out = sv;
+        a.xn = sx;
+        a.part_vmax = spv;
+        a.force_out = nullptr;
+        float best = 1e30f;
+        for (int r = 0; r < 4 && rc == SNH_OK; r++) {
+            (void)hipEventRecord(e0, c->stream);
+            launch_node_fp64(c, a);
+            (void)hipEventRecord(e1, c->stream);
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+                rc = set_err(SNH_EHIP, "autotune node launch failed");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0 && ms < best) best = ms;   /* the first launch is a warm-up */
+        }
+        rep.node_ms[o] = best;
+    }
+    c->node_variant = saved_v;
+    c->node_kc = saved_kc;
+    c->n_tile_blocks = saved_tb;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    void* p[] = {sv, sx, spv};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    if (rc) return rc;
+    int best = 0;
+    for (int o = 1; o < SNH_NODE_TUNES; o++)
+        if (rep.node_ms[o] < rep.node_ms[best]) best = o;
+    const int kc = best == 0 ? saved_kc : kcs[best - 1];
+    const int tb = tile_blocks(c, kc);
+    if (best > 0 && tb > std::max(c->n_node_blocks, saved_tb)) {
+        /* more block maxima than the context holds room for */
+        double* pv = nullptr;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMalloc((void**)&pv, (size_t)tb * sizeof(double)));
+        (void)hipFree(c->part_vmax);
+        c->part_vmax = pv;
+    }
+    c->node_variant = best == 0 ? SNH_NODE_PER_NODE : SNH_NODE_TILED;
+    c->node_kc = kc;
+    c->n_tile_blocks = tb;
+    rep.node_variant = c->node_variant;
+    rep.node_kc = kc;
+    return SNH_OK;
+}
+
 extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
     Range range_("snh_autotune");
     if (!c) return set_err(SNH_EINVAL, "null context");
@@ -3987,6 +4060,10 @@ extern "C" int snh_autotune(snh_context* c, snh_tune_report* out) {
     c->elem_variant = best;
     if (getenv("SNH_ELEM_VARIANT")) c->elem_variant = saved;
     rep.elem_variant = c->elem_variant;
+    rep.node_variant = c->node_variant;
+    rep.node_kc = c->node_kc;
+    if (c->cfg.precision == SNH_PRECISION_FP64 && !getenv("SNH_NODE_VARIANT") && (rc = autotune_node(c, rep)))
+        return rc;
     if (out) *out = rep;
     return SNH_OK;
 }

@@ -338,3 +338,50 @@ def test_wave_speed_of_sound_is_a_running_maximum():
     after = c.scalars()["speedOfSound"]
     c.Destroy()
     assert before == after == pytest.approx((1.8e11 / 2500.0) ** 0.5, rel=1e-6)
+
+
+@pytest.mark.parametrize("kc", ["4", "16"])
+def test_tiled_node_pass_tile_heights_match_the_oracle(kc, monkeypatch):
+    """The tiled node pass with 4 and 16 node planes per workgroup (the
+    autotuner's other tile heights), against the oracle."""
+    from snac_amd import Context, _abi
+    from oracle.oracle import Oracle
+    monkeypatch.setenv("SNH_NODE_KC", kc)
+    p = P.mohr_coulomb_extension(elements=(11, 7, 19), seed=4)
+    n = 20
+    o = Oracle(p, threads=4)
+    o.hydrostatic_ic()
+    o.timestep_zero()
+    o.step(n)
+    ref, rsc = o.state(), o.scalars()
+    o.close()
+    c = Context(p).Construct().Build().Initialise()
+    c.set_node_variant(_abi.NODE_TILED)
+    c.Execute(n)
+    got, gsc = c.state(fields=FIELDS), c.scalars()
+    c.Destroy()
+    _check(f"tiled kc {kc}", ref, got, rsc, gsc)
+
+
+def test_autotuned_context_matches_the_oracle():
+    """snh_autotune times the element layouts and the node pass (per node,
+    tiled with 4, 8, 16 planes) on the context's state and keeps the
+    fastest; the steps after it are the oracle's, bit for bit."""
+    from snac_amd import Context
+    from oracle.oracle import Oracle
+    p = P.mohr_coulomb_shear(elements=(9, 5, 13))
+    c = Context(p).Construct().Build().Initialise()
+    rep = c.autotune()
+    print(f"  autotune: {rep}", flush=True)
+    assert all(rep[k] > 0 for k in ("node_ms_per_node", "node_ms_tiled_kc4", "node_ms_tiled_kc8",
+                                    "node_ms_tiled_kc16"))
+    assert rep["node_kc"] in (4, 8, 16)
+    c.Execute(20)
+    got, gsc = c.state(fields=FIELDS), c.scalars()
+    c.Destroy()
+    o = Oracle(p, threads=4)
+    o.timestep_zero()
+    o.step(20)
+    ref, rsc = o.state(), o.scalars()
+    o.close()
+    _check("autotuned", ref, got, rsc, gsc)
