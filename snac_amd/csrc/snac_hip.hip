@@ -633,6 +633,7 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
     constexpr int NT = TW ? NTET + 1 : NTET; /* padded against LDS bank conflicts */
     __shared__ double sh_a[NE][NT];
     __shared__ double sh_b[NE][NT];
+    __shared__ double sh_c[NE][NT];   /* the stress-trace sums */
     __shared__ double sh_sp[NE][NT][5];   /* matched, phi, psi, coh, hard */
     __shared__ double sh_ps[NE][NT];
     const int lane = threadIdx.x;
@@ -831,12 +832,12 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
 
     /* stress trace averaging (Stress.c:87-137) */
     const double tr = snh_div3(st[0] + st[1] + st[2]);
-    __syncthreads();
-    if (in_wave) sh_b[le][t] = tr * k.V;
+    /* its own array: no wait for the readers of sh_b */
+    if (in_wave) sh_c[le][t] = tr * k.V;
     __syncthreads();
     double pst = 0.0;
 #pragma unroll
-    for (int q = 0; q < NTET; q++) pst += sh_b[le][q];
+    for (int q = 0; q < NTET; q++) pst += sh_c[le][q];
     const double avgTrace = pst / elemVolume;
     st[0] += -1.0 * tr + avgTrace;
     st[1] += -1.0 * tr + avgTrace;

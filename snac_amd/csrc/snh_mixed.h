@@ -88,6 +88,7 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
     constexpr int NT = TW ? NTET + 1 : NTET;
     __shared__ double sh_a[NE][NT];
     __shared__ double sh_b[NE][NT];
+    __shared__ double sh_c[NE][NT];   /* the stress-trace sums */
     __shared__ double sh_sp[NE][NT][5];
     __shared__ double sh_ps[NE][NT];
     const int lane = threadIdx.x;
@@ -284,12 +285,12 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
 
     /* stress trace averaging in FP64 (Stress.c:87-137) */
     const double tr = (st[0] + st[1] + st[2]) * (1.0 / 3.0);
-    __syncthreads();
-    if (in_wave) sh_b[le][t] = tr * (double)k.V;
+    /* its own array: no wait for the readers of sh_b */
+    if (in_wave) sh_c[le][t] = tr * (double)k.V;
     __syncthreads();
     double pst = 0.0;
 #pragma unroll
-    for (int q = 0; q < NTET; q++) pst += sh_b[le][q];
+    for (int q = 0; q < NTET; q++) pst += sh_c[le][q];
     const double avgTrace = pst * rvol;
     st[0] += -tr + avgTrace;
     st[1] += -tr + avgTrace;
