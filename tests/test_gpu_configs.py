@@ -51,6 +51,36 @@ def test_adjoint_configuration_500_steps(make):
     _check_equal(f"{p.name} 64x32x64, 500 steps", ref, got, rsc, gsc)
 
 
+def _viscoplastic_weak_crust():
+    p = P.mohr_coulomb_weak_crust(elements=(64, 32, 64))
+    m = p.materials[0]
+    m.rheology = _abi.RHEOLOGY_VISCOPLASTIC
+    m.refvisc = 1.0e21
+    m.activationE = 0.0
+    m.srexponent = 1.0
+    p.name = "viscoplastic_weak_crust"
+    return p
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("make", [_viscoplastic_weak_crust, lambda: P.maxwell_thermal(elements=(64, 32, 64))],
+                         ids=["viscoplastic", "maxwell_thermal"])
+def test_other_laws_at_the_adjoint_size_200_steps(make):
+    """The visco-plastic law (weak crust, yielding) and the temperature
+    plugin (Maxwell, linear viscosity) at 64 x 32 x 64 elements over 200
+    steps, bit for bit."""
+    p = make()
+    fields = FIELDS + (("temperature",) if p.thermal is not None else ())
+    ref, got, rsc, gsc = run_both(p, 200, threads=16, fields=fields)
+    for f in fields:
+        d = np.max(np.abs(ref[f] - got[f]))
+        print(f"  {p.name} 64x32x64, 200 steps {f}: max |gpu - oracle| = {d:.3e}", flush=True)
+        assert np.array_equal(ref[f], got[f]), f
+    assert rsc["dt"] == gsc["dt"]
+    if p.thermal is None:
+        assert np.any(ref["plastic_strain"] > 0.0)
+
+
 def _topography(p, x0, x):
     I, J, K = p.nodes
     top = (np.arange(p.n_nodes) // I) % J == J - 1
