@@ -5,6 +5,8 @@ oracle on the same inputs, bit for bit):
       1000 steps (the reference's CPU-runnable case);
   Mohr-Coulomb extension, 256 x 64 x 128 elements (2.1 M), the full mesh of
       the headline benchmark, 5 steps (the oracle needs ~0.4 s per step);
+  Maxwell, 192 x 48 x 192 elements (1.8 M), the mixed-precision
+      configuration's mesh at FP64, 5 steps;
   Maxwell (log-viscosity field) and Mohr-Coulomb weak crust, 64 x 32 x 64
       elements, 500 steps (the adjoint's configuration);
 and the mixed-precision error at the Maxwell configuration's full size,
@@ -40,6 +42,16 @@ def test_mohr_coulomb_extension_full_mesh():
     p = P.mohr_coulomb_extension()
     ref, got, rsc, gsc = run_both(p, 5, threads=16)
     _check_equal("mohr_coulomb 256x64x128, 5 steps", ref, got, rsc, gsc)
+
+
+@pytest.mark.timeout(600)
+def test_maxwell_full_mesh():
+    """The mixed-precision configuration's own mesh, 192 x 48 x 192 Maxwell
+    elements with the log-normal viscosity field, in FP64 against the
+    oracle, 5 steps."""
+    p = P.maxwell_viscoelastic()
+    ref, got, rsc, gsc = run_both(p, 5, threads=16)
+    _check_equal("maxwell 192x48x192, 5 steps", ref, got, rsc, gsc)
 
 
 @pytest.mark.parametrize("make", [lambda: P.maxwell_viscoelastic(elements=(64, 32, 64)),
