@@ -546,6 +546,11 @@ def main():
         out["roofline"]["valu_issue_frac"] = traffic["k_" + dom].get("valu_issue_frac")
     if traffic:
         out["counters"] = traffic
+        # HBM bytes the counters saw per element-step over the step's kernels
+        # (k_fin, one block, is left out), beside the compulsory bytes
+        measured = sum(traffic[k]["bytes"] for k in ("k_elem", "k_node", "k_geo") if k in traffic)
+        out["step_roofline"]["measured_bytes_per_element_step"] = measured / E
+        out["step_roofline"]["measured_over_compulsory"] = measured / E / step_bytes
 
     if not args.no_adjoint and args.precision == "fp64":
         from snac_amd import problem as P
