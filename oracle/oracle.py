@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
-checker) and bench.py's CPU-baseline leg.  The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (to
+compare against) and bench.py's CPU-baseline leg.  The product never imports it.
 """
 from __future__ import annotations
 

@@ -2,7 +2,7 @@
  * snac_oracle.h -- CPU oracle for SNAC's explicit time step.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
- * algorithm (geodynamics/snac), used by tests/ as the checker of the HIP path
+ * algorithm (geodynamics/snac), used by tests/ as the reference of the HIP path
  * and by bench.py as the timed CPU baseline.  The product (snac_amd/, the C ABI
  * in include/snac_hip.h) never links, loads or calls it.
  *

@@ -1,8 +1,9 @@
 /*
  * tangent.cpp -- FP64 tangent-linear model of SNAC's explicit step on the CPU.
  *
- * TEST INFRASTRUCTURE ONLY: the checker of the GPU adjoint (include/
- * snac_hip.h, snh_adjoint_*).  Nothing in the product links or calls it.
+ * TEST INFRASTRUCTURE ONLY: the reference the GPU adjoint (include/
+ * snac_hip.h, snh_adjoint_*) is tested against.  Nothing in the product links
+ * or calls it.
  *
  * It restates the step of the reference in the order of AbstractContext_Step
  * (Context.c:1063-1268): strain rate (StrainRate.c:65-305), strain and

@@ -1,6 +1,6 @@
 """ctypes binding of the FP64 tangent-linear model (libtangent.so).
 
-TEST INFRASTRUCTURE ONLY: the checker of the GPU adjoint.  The product never
+TEST INFRASTRUCTURE ONLY: the reference of the GPU adjoint.  The product never
 imports it.
 
 record() runs the CPU oracle from the problem's initial state and keeps what

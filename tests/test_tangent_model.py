@@ -1,4 +1,4 @@
-"""The FP64 tangent-linear model (oracle/tangent.cpp), the checker of the GPU
+"""The FP64 tangent-linear model (oracle/tangent.cpp), the reference of the GPU
 adjoint, pinned on the CPU: its values follow the CPU oracle's trajectory,
 and its dual-number derivative equals central finite differences of the same
 map with the discrete decisions frozen (dt, damping branch, plastic branch
