@@ -415,7 +415,8 @@ def hbm_traffic(args, kernel_prefixes=("k_elem", "k_node", "k_geo")):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000,
+                    help="timed steps (default: the Mohr-Coulomb extension's full 2000-step run)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="mohr_coulomb_extension")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="length of the CPU oracle sample")
