@@ -438,8 +438,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding over several GPUs is designed, not built).", file=sys.stderr)
+        print(f"bench.py: the step runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
+              "(the split over several GPUs is designed, not implemented).", file=sys.stderr)
         sys.exit(2)
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(64, args.ensemble_steps)}))
