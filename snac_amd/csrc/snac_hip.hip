@@ -651,6 +651,19 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
     const double dt_rot = a.scal->dt_rot;
     const DevMaterial& m = mat_of<MM>(a, ec);
 
+    /* With the Maxwell law alone, the tet's stored stress and the element's
+       viscosity are loaded ahead of the kinematics and the block barrier, so
+       that their latency overlaps them (3 % off the pass; with the other
+       laws the same order measured 1-3 % slower). */
+    constexpr bool kEarly = RSET == RS_MAXWELL;
+    double st[6];
+    double rvisc_e = 0.0;
+    if constexpr (kEarly) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) st[c] = active ? a.stress[((size_t)c * NTET + t) * E + ec] : 0.0;
+        rvisc_e = a.refvisc ? a.refvisc[ec] : m.refvisc;
+    }
+
     /* kinematics of this lane's tet (UpdateElement.c:84-219, StrainRate.c:65-184) */
     int nd[4];
     double Xt[4][3], Ut[4][3];
@@ -683,9 +696,10 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
     const double srcorr = partial / elemVolume;
 
     /* rotation of the stored stress, strain (Stress.c:71-82) */
-    double st[6];
+    if constexpr (!kEarly) {
 #pragma unroll
-    for (int c = 0; c < 6; c++) st[c] = active ? a.stress[((size_t)c * NTET + t) * E + ec] : 0.0;
+        for (int c = 0; c < 6; c++) st[c] = active ? a.stress[((size_t)c * NTET + t) * E + ec] : 0.0;
+    }
     rotate_stress(st, k.w, dt_rot);
     double sn[6];
 #pragma unroll
@@ -781,7 +795,7 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
         double trace_strain = sn[0] + sn[1] + sn[2];
         double d0, d1, d2;
         devstrain(sn, trace_strain, d0, d1, d2);
-        double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
+        double rvisc = kEarly ? rvisc_e : (a.refvisc ? a.refvisc[ec] : m.refvisc);
         double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt, tet_temp(a, m, nd));
         if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
         maxwell_update(m, st, sn, trace_strain, visc, dt);
@@ -814,7 +828,7 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
         add_thermal_stress(a, st, t, ec);
         double d0, d1, d2;
         devstrain(sn, trace_strain, d0, d1, d2);
-        double rvisc = a.refvisc ? a.refvisc[ec] : m.refvisc;
+        double rvisc = kEarly ? rvisc_e : (a.refvisc ? a.refvisc[ec] : m.refvisc);
         double visc = viscosity_of(m, rvisc, d0, d1, d2, sn, dt, tet_temp(a, m, nd));
         if (a.visc && active) a.visc[(size_t)t * E + ec] = visc;
         maxwell_update(m, st, sn, trace_strain, visc, dt);
