@@ -2250,8 +2250,24 @@ __global__ void __launch_bounds__(GEO_BLOCK) k_geo(StepArgs a) { geo_pass(a, (in
 __device__ __forceinline__ void fin_pass(StepArgs a, const int nmin, const int nvmax) {
     __shared__ double sh[FIN_BLOCK / 64];
     double mn = INFINITY, mx = -INFINITY;
-    for (int q = threadIdx.x; q < nmin; q += blockDim.x) mn = fmin(mn, a.part_min[q]);
-    for (int q = threadIdx.x; q < nvmax; q += blockDim.x) mx = fmax(mx, a.part_vmax[q]);
+    /* eight partials in flight per thread: indices past the end read the
+       last partial again, which leaves a minimum or maximum unchanged (and
+       neither depends on the order) */
+    const int nb = (int)blockDim.x;
+    for (int q0 = threadIdx.x; q0 < nmin; q0 += 8 * nb) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = a.part_min[min(q0 + u * nb, nmin - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) mn = fmin(mn, v[u]);
+    }
+    for (int q0 = threadIdx.x; q0 < nvmax; q0 += 8 * nb) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = a.part_vmax[min(q0 + u * nb, nvmax - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) mx = fmax(mx, v[u]);
+    }
     mn = block_min(mn, sh);
     __syncthreads();
     mx = block_max(mx, sh);
