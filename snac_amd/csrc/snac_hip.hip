@@ -663,6 +663,12 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
         for (int c = 0; c < 6; c++) st[c] = active ? a.stress[((size_t)c * NTET + t) * E + ec] : 0.0;
         rvisc_e = a.refvisc ? a.refvisc[ec] : m.refvisc;
     }
+    /* the plastic laws' stored plastic strain, likewise ahead of the
+       kinematics (1.6 % off the Mohr-Coulomb pass) */
+    const bool is_pl = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC);
+    const bool is_vp = (RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC);
+    double ps = 0.0;
+    if ((RSET & (RS_PLASTIC | RS_VISCOPLASTIC)) && (is_pl || is_vp) && active) ps = a.ps[(size_t)t * E + ec];
 
     /* kinematics of this lane's tet (UpdateElement.c:84-219, StrainRate.c:65-184) */
     int nd[4];
@@ -707,13 +713,9 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
 #pragma unroll
     for (int c = 3; c < 6; c++) sn[c] = k.sr[c] * dt;
 
-    const bool is_pl = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC);
-    const bool is_vp = (RSET & RS_VISCOPLASTIC) && (m.rheology & SNH_RHEOLOGY_VISCOPLASTIC);
-    double ps = 0.0;
     Soft sp = {0.0, 0.0, 0.0, 0.0};
     if (RSET & (RS_PLASTIC | RS_VISCOPLASTIC)) {
         /* own segment match, then resolve the carried parameters in tet order */
-        if ((is_pl || is_vp) && active) ps = a.ps[(size_t)t * E + ec];
         double matched = 0.0;
         Soft own = {0.0, 0.0, 0.0, 0.0};
         if (is_pl || is_vp) {
