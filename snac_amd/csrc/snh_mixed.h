@@ -107,6 +107,10 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
     /* the element's viscosity, loaded ahead of the kinematics and the block
        barriers so that its latency overlaps them (1.5 % off the Maxwell pass) */
     const double rvisc_e = (RSET & RS_MAXWELL) && a.refvisc ? a.refvisc[ec] : m.refvisc;
+    /* and the stored plastic strain */
+    const bool is_pl = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC);
+    double ps = 0.0;
+    if ((RSET & RS_PLASTIC) && is_pl && active) ps = a.ps[(size_t)t * E + ec];
 
     int nd[4];
     double X0[3];
@@ -155,11 +159,8 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
 #pragma unroll
     for (int c = 3; c < 6; c++) sn[c] = k.sr[c] * dtf;
 
-    const bool is_pl = (RSET & RS_PLASTIC) && (m.rheology & SNH_RHEOLOGY_PLASTIC);
-    double ps = 0.0;
     Soft sp = {0.0, 0.0, 0.0, 0.0};
     if (RSET & RS_PLASTIC) {
-        if (is_pl && active) ps = a.ps[(size_t)t * E + ec];
         double matched = 0.0;
         Soft own = {0.0, 0.0, 0.0, 0.0};
         if (is_pl) {
