@@ -1269,10 +1269,10 @@ __device__ __forceinline__ int gather_node(const StepArgs& a, int g) {
    undamped force, the mass and the damping branch.  Returns |v| for a
    constrained node, -inf otherwise (the Sync's vmax, Context.c:1218-1235). */
 __device__ __forceinline__ double node_finish(const StepArgs& a, int n, double f0, double f1, double f2, double im,
-                                              double x0, double x1, double x2, double dt) {
+                                              double x0, double x1, double x2, double v0, double v1, double v2,
+                                              double dt) {
     double vmag = -INFINITY;
     const size_t N = (size_t)a.N;
-    double v0 = a.v[n], v1 = a.v[N + n], v2 = a.v[2 * N + n];
     if (a.tape_f) {
         /* what the reverse pass needs: the undamped force and the mass */
         a.tape_f[n] = f0;
@@ -1338,6 +1338,11 @@ __device__ __forceinline__ void node_pass(StepArgs a, const int blk, const int n
         const double dt = a.scal->dt;
         const double c = a.scal->speedOfSound;
         const size_t E = (size_t)a.E;
+        /* the node's own coordinates and velocity, loaded ahead of the
+           gather so that their latency overlaps it */
+        const size_t N = (size_t)a.N;
+        const double x0 = a.xc[n], x1 = a.xc[N + n], x2 = a.xc[2 * N + n];
+        const double u0 = a.v[n], u1 = a.v[N + n], u2 = a.v[2 * N + n];
         double im = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
 #pragma unroll
         for (int slot = 0; slot < 8; slot++) {
@@ -1392,7 +1397,7 @@ __device__ __forceinline__ void node_pass(StepArgs a, const int blk, const int n
                 f1 -= a.gravity * rho * V / 8.0;
             }
         }
-        vmag = node_finish(a, n, f0, f1, f2, im, a.xc[n], a.xc[(size_t)a.N + n], a.xc[2 * (size_t)a.N + n], dt);
+        vmag = node_finish(a, n, f0, f1, f2, im, x0, x1, x2, u0, u1, u2, dt);
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blk] = bm;
@@ -1636,7 +1641,8 @@ __global__ void __launch_bounds__(NT_THREADS) k_node_tile(StepArgs a) {
             const int sl = k & 1;
             const size_t n = (size_t)i + (size_t)a.I * ((size_t)j + (size_t)a.J * k);
             const double v = node_finish(a, (int)n, acc[0], acc[1], acc[2], acc[3], sx[sl][0][tj + 1][ti + 1],
-                                         sx[sl][1][tj + 1][ti + 1], sx[sl][2][tj + 1][ti + 1], dt);
+                                         sx[sl][1][tj + 1][ti + 1], sx[sl][2][tj + 1][ti + 1], a.v[n],
+                                         a.v[(size_t)a.N + n], a.v[2 * (size_t)a.N + n], dt);
             vmag = fmax(vmag, v);
         }
 #pragma unroll

@@ -391,7 +391,8 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
         if (a.dtType != SNH_DT_DYNAMIC) im = 1.0; /* host rejects other dt types in mixed mode */
         /* damping (or the prescribed damping branch), velocity, BC,
            coordinates: the FP64 node update */
-        vmag = node_finish(a, n, f0, f1, f2, im, xn0, xn1, xn2, dt);
+        vmag = node_finish(a, n, f0, f1, f2, im, xn0, xn1, xn2, a.v[n], a.v[(size_t)a.N + n], a.v[2 * (size_t)a.N + n],
+                           dt);
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blk] = bm;
