@@ -1263,16 +1263,46 @@ __device__ __forceinline__ int gather_node(const StepArgs& a, int g) {
     return i + a.I * (j + a.J * k);
 }
 
+/* a node's own state for its update: coordinates, velocity and velocity
+   condition, loaded ahead of its gather so that their latency overlaps it */
+struct NodeOwn {
+    double x0, x1, x2, v0, v1, v2;
+    uint32_t code, mask;
+    double b0, b1, b2;
+};
+__device__ __forceinline__ NodeOwn node_own(const StepArgs& a, int n) {
+    const size_t N = (size_t)a.N;
+    NodeOwn s;
+    s.x0 = a.xc[n];
+    s.x1 = a.xc[N + n];
+    s.x2 = a.xc[2 * N + n];
+    s.v0 = a.v[n];
+    s.v1 = a.v[N + n];
+    s.v2 = a.v[2 * N + n];
+    s.code = a.bc[n];
+    s.mask = 0u;
+    s.b0 = s.b1 = s.b2 = 0.0;
+    if (s.code) {
+        const DevBC& b = a.bcs[s.code];
+        s.mask = b.mask;
+        s.b0 = b.val[0];
+        s.b1 = b.val[1];
+        s.b2 = b.val[2];
+    }
+    return s;
+}
+
 /* the second node pass for node n (UpdateNode.c:60-158) once its force f
    and inertial mass im are gathered: damping, velocity, velocity BC and
    coordinates (x: the node's current coordinates); with a replay tape, the
    undamped force, the mass and the damping branch.  Returns |v| for a
    constrained node, -inf otherwise (the Sync's vmax, Context.c:1218-1235). */
 __device__ __forceinline__ double node_finish(const StepArgs& a, int n, double f0, double f1, double f2, double im,
-                                              double x0, double x1, double x2, double v0, double v1, double v2,
-                                              double dt) {
+                                              const NodeOwn& own, double dt) {
     double vmag = -INFINITY;
     const size_t N = (size_t)a.N;
+    const double x0 = own.x0, x1 = own.x1, x2 = own.x2;
+    double v0 = own.v0, v1 = own.v1, v2 = own.v2;
     if (a.tape_f) {
         /* what the reverse pass needs: the undamped force and the mass */
         a.tape_f[n] = f0;
@@ -1306,12 +1336,10 @@ __device__ __forceinline__ double node_finish(const StepArgs& a, int n, double f
     v0 += dt * f0 / im;
     v1 += dt * f1 / im;
     v2 += dt * f2 / im;
-    const uint32_t code = a.bc[n];
-    if (code) {
-        const DevBC& b = a.bcs[code];
-        if (b.mask & 1u) v0 = b.val[0];
-        if (b.mask & 2u) v1 = b.val[1];
-        if (b.mask & 4u) v2 = b.val[2];
+    if (own.code) {
+        if (own.mask & 1u) v0 = own.b0;
+        if (own.mask & 2u) v1 = own.b1;
+        if (own.mask & 4u) v2 = own.b2;
         vmag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
     }
     /* a NaN or Inf in stress, mass or force reaches the velocities within
@@ -1338,11 +1366,7 @@ __device__ __forceinline__ void node_pass(StepArgs a, const int blk, const int n
         const double dt = a.scal->dt;
         const double c = a.scal->speedOfSound;
         const size_t E = (size_t)a.E;
-        /* the node's own coordinates and velocity, loaded ahead of the
-           gather so that their latency overlaps it */
-        const size_t N = (size_t)a.N;
-        const double x0 = a.xc[n], x1 = a.xc[N + n], x2 = a.xc[2 * N + n];
-        const double u0 = a.v[n], u1 = a.v[N + n], u2 = a.v[2 * N + n];
+        const NodeOwn own = node_own(a, n);
         double im = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
 #pragma unroll
         for (int slot = 0; slot < 8; slot++) {
@@ -1397,7 +1421,7 @@ __device__ __forceinline__ void node_pass(StepArgs a, const int blk, const int n
                 f1 -= a.gravity * rho * V / 8.0;
             }
         }
-        vmag = node_finish(a, n, f0, f1, f2, im, x0, x1, x2, u0, u1, u2, dt);
+        vmag = node_finish(a, n, f0, f1, f2, im, own, dt);
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blk] = bm;
@@ -1640,9 +1664,7 @@ __global__ void __launch_bounds__(NT_THREADS) k_node_tile(StepArgs a) {
         if (active) {
             const int sl = k & 1;
             const size_t n = (size_t)i + (size_t)a.I * ((size_t)j + (size_t)a.J * k);
-            const double v = node_finish(a, (int)n, acc[0], acc[1], acc[2], acc[3], sx[sl][0][tj + 1][ti + 1],
-                                         sx[sl][1][tj + 1][ti + 1], sx[sl][2][tj + 1][ti + 1], a.v[n],
-                                         a.v[(size_t)a.N + n], a.v[2 * (size_t)a.N + n], dt);
+            const double v = node_finish(a, (int)n, acc[0], acc[1], acc[2], acc[3], node_own(a, (int)n), dt);
             vmag = fmax(vmag, v);
         }
 #pragma unroll

@@ -325,8 +325,8 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
         const double c = a.scal->speedOfSound;
         const double rc2 = 1.0 / (c * c);
         const size_t E = (size_t)a.E;
-        const size_t NN = (size_t)a.N;
-        const double xn0 = a.xc[n], xn1 = a.xc[NN + n], xn2 = a.xc[2 * NN + n];
+        const NodeOwn own = node_own(a, n);
+        const double xn0 = own.x0, xn1 = own.x1, xn2 = own.x2;
         double im = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
 #pragma unroll
         for (int slot = 0; slot < 8; slot++) {
@@ -391,8 +391,7 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
         if (a.dtType != SNH_DT_DYNAMIC) im = 1.0; /* host rejects other dt types in mixed mode */
         /* damping (or the prescribed damping branch), velocity, BC,
            coordinates: the FP64 node update */
-        vmag = node_finish(a, n, f0, f1, f2, im, xn0, xn1, xn2, a.v[n], a.v[(size_t)a.N + n], a.v[2 * (size_t)a.N + n],
-                           dt);
+        vmag = node_finish(a, n, f0, f1, f2, im, own, dt);
     }
     double bm = block_max(vmag, sh);
     if (threadIdx.x == 0) a.part_vmax[blk] = bm;
