@@ -62,6 +62,20 @@ __device__ __forceinline__ int node_index(const StepArgs& a, int i, int j, int k
     return i + a.I * (j + a.J * k);
 }
 
+/* (ei, ej, ek) of element e and (i, j, k) of node n */
+__device__ __forceinline__ void elem_ijk(const StepArgs& a, int e, int& ei, int& ej, int& ek) {
+    const int q = (int)fastdiv((uint32_t)e, a.dEI);
+    ek = (int)fastdiv((uint32_t)q, a.dEJ);
+    ei = e - q * a.EI;
+    ej = q - ek * a.EJ;
+}
+__device__ __forceinline__ void node_ijk(const StepArgs& a, int n, int& i, int& j, int& k) {
+    const int q = (int)fastdiv((uint32_t)n, a.dI);
+    k = (int)fastdiv((uint32_t)q, a.dJ);
+    i = n - q * a.I;
+    j = q - k * a.J;
+}
+
 /* load the 8 hex nodes of element (i,j,k): coords and velocities */
 __device__ __forceinline__ void load_hex(const StepArgs& a, const double* __restrict__ x, int i, int j, int k,
                                          double X[8][3]) {
@@ -646,7 +660,8 @@ __device__ __forceinline__ void elem_pass(StepArgs a, const int blk, const int n
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
-    const int ei = ec % a.EI, ej = (ec / a.EI) % a.EJ, ek = ec / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, ec, ei, ej, ek);
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
     const DevMaterial& m = mat_of<MM>(a, ec);
@@ -880,7 +895,8 @@ template <int RSET, bool LOWREG, bool MM>
 __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
-    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, e, ei, ej, ek);
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
     const DevMaterial& m = mat_of<MM>(a, e);
@@ -1243,16 +1259,19 @@ __global__ void __launch_bounds__(GEO_BLOCK) k_density(StepArgs a) {
    then run close together in time on one XCD and share its L2, where the
    plain order puts a whole k-plane (J rows) between them.  node_band = 0
    keeps the plain order.  A bijection of [0, N). */
-__device__ __forceinline__ int gather_node(const StepArgs& a, int g) {
+__device__ __forceinline__ int gather_node(const StepArgs& a, int g, int& i, int& j, int& k) {
     const int KB = a.node_band;
-    if (KB <= 0) return g;
-    const int r = g / a.I, i = g - r * a.I;
+    if (KB <= 0) {
+        node_ijk(a, g, i, j, k);
+        return g;
+    }
+    const int r = (int)fastdiv((uint32_t)g, a.dI);
+    i = g - r * a.I;
     const int per = a.J * KB;
-    const int nfull = a.K / KB;
-    int j, k;
+    const int nfull = (int)fastdiv((uint32_t)a.K, a.dKB);
     if (r < nfull * per) {
-        const int b = r / per, rem = r - b * per;
-        j = rem / KB;
+        const int b = (int)fastdiv((uint32_t)r, a.dPer), rem = r - b * per;
+        j = (int)fastdiv((uint32_t)rem, a.dKB);
         k = b * KB + (rem - j * KB);
     } else {
         const int kl = a.K - nfull * KB;
@@ -1361,8 +1380,8 @@ __device__ __forceinline__ void node_pass(StepArgs a, const int blk, const int n
     const int g = xcd_block(blk, nblk) * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
     if (g < a.N) {
-        const int n = gather_node(a, g);
-        const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+        int i, j, k;
+        const int n = gather_node(a, g, i, j, k);
         const double dt = a.scal->dt;
         const double c = a.scal->speedOfSound;
         const size_t E = (size_t)a.E;
@@ -2176,8 +2195,8 @@ template <bool MM>
 __global__ void __launch_bounds__(NODE_BLOCK) k_heat_node(StepArgs a) {
     const int g = xcd_block() * blockDim.x + threadIdx.x;
     if (g >= a.N) return;
-    const int n = gather_node(a, g);
-    const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+    int i, j, k;
+    const int n = gather_node(a, g, i, j, k);
     const size_t E = (size_t)a.E;
     const double dt = a.scal->dt;
     const double minLen = a.scal->minLen;
@@ -2230,7 +2249,8 @@ __device__ __forceinline__ void geo_pass(StepArgs a, const int blk, const int nb
     const int e = xcd_block(blk, nblk) * blockDim.x + threadIdx.x;
     double emin = INFINITY;
     if (e < a.E) {
-        const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+        int ei, ej, ek;
+        elem_ijk(a, e, ei, ej, ek);
         double X[8][3];
         load_hex(a, a.xc, ei, ej, ek, X);
         emin = F_1EP27;
@@ -2370,6 +2390,7 @@ struct BatchMember {
     int64_t part_vmax;
     int64_t scal;
     int32_t I, J, K, EI, EJ, EK, N, E, dtType, node_band, node_kc;
+    FastDiv dI, dJ, dEI, dEJ, dKB, dPer;   /* StepArgs' divisors */
     double gravity, demf, strain_inert, c_wave;
 };
 
@@ -2435,6 +2456,12 @@ __device__ __forceinline__ StepArgs member_args(const StepArgs& base, const Batc
     a.demf = d.demf;
     a.strain_inert = d.strain_inert;
     a.c_wave = d.c_wave;
+    a.dI = d.dI;
+    a.dJ = d.dJ;
+    a.dEI = d.dEI;
+    a.dEJ = d.dEJ;
+    a.dKB = d.dKB;
+    a.dPer = d.dPer;
     return a;
 }
 
@@ -2736,6 +2763,16 @@ static bool on_wall(const snh_context* c, int n, int wall) {
     }
 }
 
+/* the multiply-high divisors of a StepArgs' index arithmetic */
+static void set_divisors(StepArgs& a) {
+    a.dI = fastdiv_make((uint32_t)a.I);
+    a.dJ = fastdiv_make((uint32_t)a.J);
+    a.dEI = fastdiv_make((uint32_t)(a.EI > 0 ? a.EI : 1));
+    a.dEJ = fastdiv_make((uint32_t)(a.EJ > 0 ? a.EJ : 1));
+    a.dKB = fastdiv_make((uint32_t)(a.node_band > 0 ? a.node_band : 1));
+    a.dPer = fastdiv_make((uint32_t)(a.node_band > 0 ? a.J * a.node_band : 1));
+}
+
 static StepArgs make_args(snh_context* c) {
     StepArgs a;
     a.xc = c->x[c->cur];
@@ -2781,6 +2818,7 @@ static StepArgs make_args(snh_context* c) {
     a.demf = c->cfg.demf;
     a.strain_inert = c->cfg.strain_inert;
     a.c_wave = c->c_wave;
+    set_divisors(a);
     return a;
 }
 
@@ -4257,6 +4295,7 @@ extern "C" int snh_sync_state(const snh_config* cfg, const double* coord, const 
     a.strain_inert = cfg->strain_inert;
     a.c_wave = 0.0;
     for (uint32_t q = 0; q < cfg->nmaterials && cfg->materials; q++) a.c_wave = wave_speed_max(a.c_wave, cfg->materials[q]);
+    set_divisors(a);
     hipStream_t st = (hipStream_t)stream;
     const int ngeo = (int)((E + GEO_BLOCK - 1) / GEO_BLOCK), nnode = (int)((N + NODE_BLOCK - 1) / NODE_BLOCK);
     hipLaunchKernelGGL(k_sync_init, dim3(1), dim3(64), 0, st, a.scal, cfg->timeStep);
@@ -4498,6 +4537,12 @@ static int ens_batch_tables(snh_ensemble* e, EnsGroup& grp) {
             d.demf = a.demf;
             d.strain_inert = a.strain_inert;
             d.c_wave = a.c_wave;
+            d.dI = a.dI;
+            d.dJ = a.dJ;
+            d.dEI = a.dEI;
+            d.dEJ = a.dEJ;
+            d.dKB = a.dKB;
+            d.dPer = a.dPer;
         }
         HIPCHK(hipMalloc((void**)&grp.dmem[q], n * sizeof(BatchMember)));
         HIPCHK(ens_upload(e, grp.dmem[q], mem.data(), n * sizeof(BatchMember)));

@@ -83,6 +83,25 @@ struct DevBC {
     double val[3];
 };
 
+/* n / d for 0 <= n < 2^31 by a multiply-high, an add and a shift
+   (T. Granlund and P. Montgomery, "Division by invariant integers using
+   multiplication", PLDI 1994, sec. 4): l = ceil(log2 d),
+   m = floor(2^32 (2^l - d) / d) + 1 < 2^32, and n / d = (mulhi(m, n) + n) >> l,
+   which cannot overflow while n < 2^31.  Replaces the ~15-instruction
+   generic division for the mesh's index arithmetic. */
+struct FastDiv {
+    uint32_t m, l;
+};
+SNH_HD FastDiv fastdiv_make(uint32_t d) {
+    uint32_t l = 0;
+    while (((uint64_t)1 << l) < d) l++;
+    FastDiv f;
+    f.m = (uint32_t)(((((uint64_t)1 << l) - d) << 32) / d + 1);
+    f.l = l;
+    return f;
+}
+__device__ __forceinline__ uint32_t fastdiv(uint32_t n, FastDiv f) { return (__umulhi(n, f.m) + n) >> f.l; }
+
 struct StepArgs {
     const double* __restrict__ xc;  /* coordinates of this step [3][N]     */
     const double* __restrict__ xp;  /* coordinates of the step before      */
@@ -139,6 +158,8 @@ struct StepArgs {
     int32_t node_kc;        /* node planes per workgroup of the tiled node pass */
     double gravity, demf, strain_inert;
     double c_wave;          /* the wave dt type's speed of sound: max Vp over the elements */
+    /* divisions by I, J, EI, EJ, node_band and J * node_band */
+    FastDiv dI, dJ, dEI, dEJ, dKB, dPer;
 };
 
 /* kinematics of one tetrahedron from its 4 node coordinates and velocities,

@@ -99,7 +99,8 @@ __device__ __forceinline__ void elem_pass_mx(StepArgs a, const int blk, const in
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
-    const int ei = ec % a.EI, ej = (ec / a.EI) % a.EJ, ek = ec / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, ec, ei, ej, ek);
     const double dt = a.scal->dt;
     const float dtf = (float)dt;
     const float dt_rotf = (float)a.scal->dt_rot;
@@ -319,8 +320,8 @@ __device__ __forceinline__ void node_pass_mx(StepArgs a, const int blk, const in
     if (g < a.N) {
         /* the FP64 gather's banded node order: node rows that read one
            element row run close together and share its L2 */
-        const int n = gather_node(a, g);
-        const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+        int i, j, k;
+        const int n = gather_node(a, g, i, j, k);
         const double dt = a.scal->dt;
         const double c = a.scal->speedOfSound;
         const double rc2 = 1.0 / (c * c);
