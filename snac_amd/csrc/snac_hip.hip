@@ -1178,7 +1178,8 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_elem_seq(StepArgs a) {
 __global__ void __launch_bounds__(ELEM_BLOCK) k_rotated(StepArgs a, double* __restrict__ out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
-    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, e, ei, ej, ek);
     const double dt_rot = a.scal->dt_rot;
     double X[8][3], U[8][3];
     load_hex(a, a.xc, ei, ej, ek, X);
@@ -1209,7 +1210,8 @@ __global__ void __launch_bounds__(ELEM_BLOCK) k_rotated(StepArgs a, double* __re
 __global__ void k_volumes(StepArgs a, double* __restrict__ out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
-    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, e, ei, ej, ek);
     double X[8][3];
     load_hex(a, a.xc, ei, ej, ek, X);
 #pragma unroll
@@ -1241,7 +1243,8 @@ __global__ void __launch_bounds__(GEO_BLOCK) k_density(StepArgs a) {
                              a.stress_out[((size_t)2 * NTET + t) * E + e]);
     pressure /= (double)NTET;
     const DevMaterial& m = mat_of<MM>(a, e);
-    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, e, ei, ej, ek);
 #pragma unroll
     for (int t = 0; t < NTET; t++) {
         double avg = 0.0;
@@ -2120,7 +2123,8 @@ template <bool MM>
 __global__ void __launch_bounds__(GEO_BLOCK) k_heat_elem(StepArgs a) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
-    const int ei = e % a.EI, ej = (e / a.EI) % a.EJ, ek = e / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, e, ei, ej, ek);
     const size_t E = (size_t)a.E;
     const DevMaterial& m = mat_of<MM>(a, e);
     double X[8][3], T[8];
@@ -4224,7 +4228,8 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_vmax_walls(StepArgs a, uint32_t 
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double vmag = -INFINITY;
     if (n < a.N) {
-        const int i = n % a.I, j = (n / a.I) % a.J, k = n / (a.I * a.J);
+        int i, j, k;
+        node_ijk(a, n, i, j, k);
         const bool on = ((walls >> SNH_WALL_ALL) & 1u) || (((walls >> SNH_WALL_FRONT) & 1u) && a.K > 1 && k == 0) ||
                         (((walls >> SNH_WALL_BACK) & 1u) && a.K > 1 && k == a.K - 1) ||
                         (((walls >> SNH_WALL_LEFT) & 1u) && a.I > 1 && i == 0) ||

@@ -449,7 +449,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8)))
     const bool active = in_wave && e < a.E;
     const size_t E = (size_t)a.E;
     const int ec = active ? e : 0;
-    const int ei = ec % a.EI, ej = (ec / a.EI) % a.EJ, ek = ec / (a.EI * a.EJ);
+    int ei, ej, ek;
+    elem_ijk(a, ec, ei, ej, ek);
     const double dt = a.scal->dt;
     const double dt_rot = a.scal->dt_rot;
     const double csound = a.scal->speedOfSound;
